@@ -1,0 +1,354 @@
+#!/usr/bin/env python3
+"""Headline benchmark of the MI355X N-body hot path.
+
+Workload (one GPU): one disk + halo galaxy, 262144 stars (mass 1.0) and
+524288 halo bodies (mass 6.699356), eps 0.004, dt 0.002, FP32 direct-sum
+gravity with FP64 predictor-corrector state.  A step is the reference's
+predictor -> ACCEL -> corrector (nbody.cu:1525-1582) over all 786432 bodies;
+`value` is whole-step pair-interactions per second (N^2 per step, i = j
+counted as the reference does), inputs resident in HBM.
+
+Also reported on the same line (one GPU):
+  roofline      gravity kernel: 20 FLOP per pair vs the 157.3 TFLOP/s FP32
+                peak, average launch time from HIP events on the library's
+                stream; traffic from rocprof counters when supplied
+  accuracy      FP32 force vs the FP64 oracle on 16384 sampled bodies
+  cpu_baseline  the FP64 oracle (OpenMP, all host cores) on those rows
+  binning       p2mesh of the final state on 256^3 over [-2,2]^3
+  stencil       200 damped-wave iterations on that grid from phi = 0
+  batched       512 Plummer spheres (N log-uniform 1024..131072) per step,
+                one persistent launch vs one plain launch per system
+  drift         E, P, L relative drift over the timed steps (FP64 device
+                reductions)
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "khrr-nbody_amd"))
+
+METRIC = "G pair-interactions/s & % FP32 VALU peak; batched/served vs plain; p99 lat"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+FLOP_PER_PAIR = 20
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-stars", type=int, default=262144)
+    ap.add_argument("--n-dm", type=int, default=524288)
+    ap.add_argument("--cpu-rows", type=int, default=16384)
+    ap.add_argument("--quick", action="store_true", help="headline only (no extras)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=2)
+    return ap.parse_args()
+
+
+class Sim:
+    """Device state of one system and the reference step sequence."""
+
+    def __init__(self, ctx, pos, vel, mass, h, dt):
+        import torch
+        dev = torch.device("cuda:0")
+        f64 = dict(device=dev, dtype=torch.float64)
+        f32 = dict(device=dev, dtype=torch.float32)
+        self.ctx, self.dt, self.n = ctx, dt, pos.shape[1]
+        self.pos = torch.tensor(pos, **f64)
+        self.vel = torch.tensor(vel, **f64)
+        self.mass64 = torch.tensor(mass, **f64)
+        self.m32 = torch.tensor(mass, **f32)
+        self.h32 = torch.tensor(h, **f32)
+        self.pos32 = self.pos.float().contiguous()
+        n = self.n
+        self.post = torch.zeros((3, n), **f64)
+        self.velt = torch.zeros((3, n), **f64)
+        self.acc = torch.zeros((3, n), **f32)
+        self.acct = torch.zeros((3, n), **f32)
+        self.phi = torch.zeros(n, **f32)
+        self.t = 0.0
+
+    def gravity(self, out, phi=None):
+        self.ctx.accel(self.pos32, self.h32, self.pos32, self.m32, self.h32, acc=out, phi=phi)
+
+    def init_forces(self):
+        self.gravity(self.acc)
+
+    def step(self):
+        c = self.ctx
+        c.integrate(0, self.dt, self.t, self.acc, self.post, self.velt, self.pos, self.vel,
+                    pos32=self.pos32)
+        self.gravity(self.acct)
+        c.integrate(1, self.dt, self.t + self.dt, self.acct, self.pos, self.vel, self.post,
+                    self.velt, acc0=self.acc)
+        self.t += self.dt
+
+    def diagnostics(self):
+        self.ctx.accel(self.pos32, self.h32, self.pos32, self.m32, self.h32, phi=self.phi)
+        return self.ctx.diag(self.pos, self.vel, self.mass64, self.phi)
+
+
+def energy(d):
+    return 0.5 * (d["Ek"] + d["Ep"])
+
+
+def bench_binning(ctx, sim, reps=5):
+    import torch
+    import khrr
+    nx = 256
+    grid = khrr.cube_grid(nx, -2.0, 2.0)
+    G = nx ** 3
+    B = (G + 511) // 512
+    n = sim.n
+    i32 = dict(device="cuda:0", dtype=torch.int32)
+    bufs = dict(cell_id=torch.zeros(n, **i32), id_in_cell=torch.zeros(n, **i32),
+                count=torch.zeros(G, **i32), offset=torch.zeros(G, **i32),
+                start_id_ref=torch.zeros(G, **i32), block_incl=torch.zeros(B, **i32),
+                cell_mass=torch.zeros(G, device="cuda:0", dtype=torch.float64))
+    ctx.p2mesh_bin(grid, sim.pos, sim.mass64, **bufs)
+    torch.cuda.synchronize()
+    ctx.reset_timing()
+    for _ in range(reps):
+        ctx.p2mesh_bin(grid, sim.pos, sim.mass64, **bufs)
+    launches, ms = ctx.kernel_time(khrr.K_BIN)
+    avg_ms = ms / max(launches, 1)
+    # algorithmic bytes: per body pos (24) + mass (8) read, cell_id (4) +
+    # id_in_cell (4) written; per cell count, offset, start_id_ref (4 each)
+    # and cell_mass (8) written
+    alg = 40 * n + 20 * G
+    return bufs["cell_mass"], {
+        "grid": "256^3 over [-2,2]^3", "n": n, "ms": avg_ms, "alg_bytes": alg,
+        "alg_GBps": alg / (avg_ms * 1e-3) / 1e9,
+        "frac_hbm_peak": alg / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+        "occupied_cells": int((bufs["count"] > 0).sum().item())}
+
+
+def bench_stencil(ctx, cell_mass, iters=200):
+    import torch
+    import khrr
+    nx = 256
+    w = khrr.reference_wave(nx, 4.0 / nx)
+    G = nx ** 3
+    src = torch.zeros(G, device="cuda:0")
+    ctx.wave_source(w, cell_mass, src)
+    b = [torch.zeros(G, device="cuda:0") for _ in range(3)]
+    ctx.wave_iterate(w, b[0], b[1], b[2], src, 5)
+    for t in b:
+        t.zero_()
+    torch.cuda.synchronize()
+    ctx.reset_timing()
+    _, cur, _ = ctx.wave_iterate(w, b[0], b[1], b[2], src, iters)
+    launches, ms = ctx.kernel_time(khrr.K_WAVE)
+    avg_ms = ms / max(launches, 1)
+    alg = 16 * G
+    return {"grid": "256^3", "iterations": iters, "ms_per_iter": avg_ms, "alg_bytes_per_iter": alg,
+            "alg_GBps": alg / (avg_ms * 1e-3) / 1e9,
+            "frac_hbm_peak": alg / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "max_abs_phi": float(cur.abs().max().item())}
+
+
+def bench_batched(ctx, steps):
+    """512 Plummer spheres: one persistent batched launch per step vs one
+    plain launch per system per step (same integrator calls)."""
+    import numpy as np
+    import torch
+    import ics
+    import khrr
+    sizes, eps = ics.plummer_population(512, 1024, 131072, seed=7)
+    pos_l, vel_l = [], []
+    for k, n in enumerate(sizes):
+        p, v = ics.plummer(int(n), 1.0, 0.3333, 1.0, seed=100000 + k)
+        pos_l.append(p)
+        vel_l.append(v)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    N = int(off[-1])
+    pos = np.concatenate(pos_l, axis=1)
+    vel = np.concatenate(vel_l, axis=1)
+    mass = np.concatenate([np.full(n, 1.0 / n) for n in sizes])
+    h = np.concatenate([np.full(n, 0.5 * e * e) for n, e in zip(sizes, eps)])
+    sim = Sim(ctx, pos, vel, mass, h, 0.002)
+    off_dev = torch.tensor(off, device="cuda:0", dtype=torch.int32)
+    off_list = off.tolist()
+    pairs = float(np.sum(sizes.astype(np.float64) ** 2))
+    tune, _ = ctx.tune_batched(off_list)
+
+    def grav_batched(out):
+        ctx.accel_batched(off_dev, off_list, sim.pos32, sim.h32, sim.m32, acc=out, tune=tune)
+
+    # each system in its own buffers, as a one-system-per-process run holds it
+    per_sys = []
+    for s in range(len(sizes)):
+        a, b = off_list[s], off_list[s + 1]
+        per_sys.append((sim.pos32[:, a:b].contiguous(), sim.h32[a:b].contiguous(),
+                        sim.m32[a:b].contiguous(), torch.zeros((3, b - a), device="cuda:0")))
+
+    def grav_plain(out):
+        del out
+        for p, hh, mm, o in per_sys:
+            ctx.accel(p, hh, p, mm, hh, acc=o)
+
+    def run(grav, k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            grav(sim.acct)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k
+
+    grav_batched(sim.acct)
+    t_b = run(grav_batched, steps)
+    log(f"batched: {t_b * 1e3:.1f} ms per force evaluation")
+    grav_plain(sim.acct)
+    t_p = run(grav_plain, max(1, steps // 2))
+    log(f"plain per-system: {t_p * 1e3:.1f} ms per force evaluation")
+    return {"systems": 512, "bodies": N, "pairs_per_step": pairs,
+            "batched_ms": t_b * 1e3, "plain_ms": t_p * 1e3,
+            "batched_Gpairs_per_s": pairs / t_b / 1e9, "plain_Gpairs_per_s": pairs / t_p / 1e9,
+            "speedup_vs_plain": t_p / t_b, "tune": tune.as_dict(),
+            "plain": "khrr_accel per system, default launch shape, one stream"}
+
+
+def cpu_baseline(sim, acc_gpu, rows_n):
+    """FP64 oracle on a bounded sample of i-rows at the final positions."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    p32 = sim.pos32.cpu().numpy().astype(np.float64)
+    m = sim.m32.cpu().numpy().astype(np.float64)
+    h = sim.h32.cpu().numpy().astype(np.float64)
+    rows = np.sort(np.random.default_rng(4096).choice(sim.n, rows_n, replace=False))
+    t0 = time.perf_counter()
+    ref, _ = orc.accel_rows(rows, p32, h, m)
+    dt = time.perf_counter() - t0
+    got = acc_gpu[:, rows].astype(np.float64)
+    e = np.linalg.norm(got - ref, axis=0) / np.linalg.norm(ref, axis=0)
+    acc = {"rows": int(rows_n), "median": float(np.median(e)),
+           "p99": float(np.percentile(e, 99)), "max": float(np.max(e)),
+           "reference": "FP64 oracle on the same FP32 inputs"}
+    pairs = rows_n * sim.n
+    cpu = {"value": pairs / dt / 1e9, "unit": "G pair-interactions/s",
+           "cores": orc.num_threads(), "kind": "port",
+           "sample": f"{rows_n} random i-rows x {sim.n} j (one force evaluation subsample, "
+                     f"FP64, OpenMP), {dt:.2f} s"}
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
+                         "")
+        cpu["cpu_model"] = model
+    except OSError:
+        pass
+    return cpu, acc
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (no multi-GPU path is built); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import ics
+    import khrr
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    log("generating galaxy")
+    ps, vs, pd, vd = ics.galaxy(args.n_stars, args.n_dm, seed=12345)
+    pos, vel, mass, h = ics.system_arrays(ps, vs, pd, vd, 1.0, 6.699356, 0.004, 0.004)
+    n = pos.shape[1]
+    ctx = khrr.Context(0)
+    sim = Sim(ctx, pos, vel, mass, h, 0.002)
+    sim.init_forces()
+    torch.cuda.synchronize()
+    log(f"n={n}; warmup {args.warmup} steps")
+    d0 = sim.diagnostics()
+    for _ in range(args.warmup):
+        sim.step()
+    torch.cuda.synchronize()
+    d_start = sim.diagnostics()
+    torch.cuda.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sim.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    grav_launches, grav_ms = ctx.kernel_time(khrr.K_GRAVITY)
+    int_launches, int_ms = ctx.kernel_time(khrr.K_INTEGRATE)
+    ctx.set_timing(False)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    pairs = float(n) * n
+    value = pairs / (ms_per_step * 1e-3) / 1e9
+    grav_avg = grav_ms / max(grav_launches, 1)
+    achieved = FLOP_PER_PAIR * pairs / (grav_avg * 1e-3) / 1e12
+    log(f"{ms_per_step:.2f} ms/step, gravity {grav_avg:.2f} ms/launch, {value:.1f} Gpairs/s")
+    d_end = sim.diagnostics()
+    drift = {"steps": args.steps,
+             "dE_rel": energy(d_end) / energy(d_start) - 1.0,
+             "dP_abs": math.sqrt(sum((d_end[k] - d_start[k]) ** 2 for k in ("Px", "Py", "Pz"))),
+             "dLz_rel": d_end["Lz"] / d_start["Lz"] - 1.0,
+             "E0": energy(d0)}
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        try:
+            traffic = json.load(open(TRAFFIC_FILE)).get("gravity_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    result = {
+        "metric": METRIC, "value": value, "unit": "G pair-interactions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "disk+halo galaxy, 262144 stars + 524288 halo, eps 0.004, "
+                               "dt 0.002, FP32 direct sum, FP64 predictor-corrector state",
+                   "n_bodies": n, "dt": 0.002, "seed": 12345},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": traffic, "kernel": "grav_kernel",
+                     "avg_launch_ms": grav_avg, "flop_per_pair": FLOP_PER_PAIR,
+                     "pairs_per_launch": pairs},
+        "integrator_ms_per_launch": int_ms / max(int_launches, 1),
+        "drift": drift,
+    }
+    if not args.no_cpu or not args.quick:
+        # fresh force at the final positions for the accuracy sample
+        chk = torch.zeros((3, n), device="cuda:0")
+        sim.gravity(chk)
+        torch.cuda.synchronize()
+        if not args.no_cpu:
+            log(f"cpu baseline: oracle on {args.cpu_rows} rows")
+            cpu, acc = cpu_baseline(sim, chk.cpu().numpy(), args.cpu_rows)
+            result["cpu_baseline"] = cpu
+            result["accuracy"] = acc
+    if not args.quick:
+        log("binning")
+        cell_mass, binning = bench_binning(ctx, sim)
+        result["binning"] = binning
+        log("stencil")
+        result["stencil"] = bench_stencil(ctx, cell_mass)
+        del cell_mass
+        torch.cuda.empty_cache()
+        log("batched")
+        result["batched"] = bench_batched(ctx, args.batched_steps)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,255 @@
+/*
+ * khrr.h -- C-ABI of the MI355X (gfx950) N-body hot path.
+ *
+ * Drop-in replacement for the device kernels of KHRR-NBody that the host
+ * driver launches through RR's CuCall(kernel, grid, block)(args...):
+ *
+ *   ACCEL_kernel            nbody/include/nbody-kernel.cuh:89-142   -> khrr_accel
+ *   PHI_kernel              nbody/include/nbody-kernel.cuh:43-86    -> khrr_accel (acc == NULL)
+ *   kernelNbody_integTime   nbody/include/nbody-kernel.cuh:144-194  -> khrr_integrate
+ *   dev_fex + DataBlock     nbody/include/nbody-kernel.cuh:8-40,
+ *                           nbody/src/nbody.cu:1111-1120            -> khrr_extpot_init
+ *   assignParticlesToCells  nbody/include/p2mesh-kernel.cuh:11-57  \
+ *   computePrefixSums       nbody/include/p2mesh-kernel.cuh:62-105  |-> khrr_p2mesh_bin
+ *   adjustGlobalPrefixSums  nbody/include/p2mesh-kernel.cuh:110-123 |   (the sequence of
+ *   computeCellMassesUnsorted p2mesh-kernel.cuh:126-137            /    nbody.cu:139-181)
+ *   wave_diss_iteration     nbody/include/wave-kernel.cuh:5-103     -> khrr_wave_step
+ *   iteration loop + swap   nbody/src/nbody.cu:227-239              -> khrr_wave_iterate
+ *   result() host sums      nbody/src/nbody.cu:797-833              -> khrr_diag
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success or a negative KHRR_E* code; the
+ *     message is available from khrr_last_error(ctx).  Nothing calls exit().
+ *   - All array arguments are DEVICE pointers owned by the caller, unless
+ *     the comment says "host".  Work is enqueued on the context's stream and
+ *     is asynchronous unless the comment says "synchronous".
+ *   - "vec3 planes": a 3-component per-particle quantity of n entries is one
+ *     buffer of 3*n elements, x in [0,n), y in [n,2n), z in [2n,3n).
+ *     (The reference stores double3 AoS; planes give coalesced SoA access.)
+ *   - Softening is passed as h = eps^2/2 per particle, so the reference's
+ *     per-pair 0.5*(eps2_i + eps2_j) (nbody-kernel.cuh:78,126) is h_i + h_j.
+ *   - Gravity arithmetic is FP32; the integrator state, binning index math,
+ *     cell masses and diagnostics are FP64 (the reference's `real`).
+ *   - A context is not re-entrant: calls on one context must not overlap
+ *     from several host threads.  Distinct contexts are independent.
+ */
+#ifndef KHRR_H
+#define KHRR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KHRR_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------- */
+#define KHRR_OK 0
+#define KHRR_EINVAL (-1)   /* bad argument (size, null pointer, flags)      */
+#define KHRR_EDEVICE (-2)  /* HIP runtime error                             */
+#define KHRR_ENOMEM (-3)   /* device allocation failed / budget exceeded    */
+#define KHRR_ESTATE (-4)   /* call not valid in the object's current state  */
+
+/* ---- flags for khrr_accel -------------------------------------------- */
+/* acc += sum, phi -= sum (read-modify-write exactly as ACCEL_kernel
+ * nbody-kernel.cuh:137-141 and PHI_kernel :85).  Without it the outputs are
+ * overwritten: acc = sum, phi = -sum. */
+#define KHRR_ACCUMULATE 1u
+/* Compensated (TwoSum) accumulation of the per-particle sums: tile partial
+ * sums are folded into a (hi, lo) pair so the FP32 result carries the
+ * rounding of the pair terms only, not of an N-term running sum. */
+#define KHRR_COMPENSATED 2u
+
+typedef struct khrr_ctx khrr_ctx;
+
+/* ---- context --------------------------------------------------------- */
+int khrr_abi_version(void);
+/* Creates a context on HIP device `device`; work goes to the null stream
+ * until khrr_ctx_set_stream is called. */
+int khrr_ctx_create(int device, khrr_ctx** out);
+void khrr_ctx_destroy(khrr_ctx* ctx);
+/* stream: a hipStream_t (NULL = the null stream). */
+int khrr_ctx_set_stream(khrr_ctx* ctx, void* stream);
+/* Last error message of this context (or of the calling thread when ctx is
+ * NULL).  Valid until the next call on the same context. */
+const char* khrr_last_error(const khrr_ctx* ctx);
+/* Blocks until all work enqueued by this context has finished. */
+int khrr_ctx_sync(khrr_ctx* ctx);
+/* Kernel timing.  When enabled, every launch of a timed kernel class is
+ * bracketed by HIP events on the context's stream.  khrr_ctx_kernel_time
+ * is synchronous and returns the launch count and summed device time (ms)
+ * since the last reset. */
+#define KHRR_K_GRAVITY 0
+#define KHRR_K_INTEGRATE 1
+#define KHRR_K_BIN 2
+#define KHRR_K_WAVE 3
+#define KHRR_K_DIAG 4
+#define KHRR_K_COUNT 5
+int khrr_ctx_set_timing(khrr_ctx* ctx, int enable);
+int khrr_ctx_kernel_time(khrr_ctx* ctx, int kernel_class, long* launches, double* total_ms);
+int khrr_ctx_reset_timing(khrr_ctx* ctx);
+
+/* ---- gravity (a1, a2) ------------------------------------------------ */
+/* Tunable launch shape of the direct-sum kernel.  Zeroed fields mean
+ * "pick automatically" (see khrr_tune_gravity). */
+typedef struct {
+  int i_per_lane;   /* i-particles held per lane: 1, 2, 4 or 8          */
+  int waves_per_block; /* 1..16                                         */
+  int j_split;      /* number of independent j-ranges (partial sums)    */
+  int j_tile;       /* j count per FP32 partial-sum tile (multiple of 16) */
+} khrr_tune;
+
+/* Softened direct sum.  For every i in [0,n_i):
+ *   d_ij = pos_j[j] - pos_i[i],  r2 = |d_ij|^2 + h_i[i] + h_j[j]
+ *   acc[i] (+)= sum_j m_j d_ij r2^-3/2        (ACCEL_kernel :89-142)
+ *   phi[i] (-)= sum_j m_j r2^-1/2, j = i incl. (PHI_kernel :43-86)
+ * pos_i: vec3 planes of n_i floats; pos_j: vec3 planes of n_j floats;
+ * acc: vec3 planes of n_i floats or NULL; phi: n_i floats or NULL (at least
+ * one of acc/phi non-NULL).  The reference requires N % 512 == 0 and reads
+ * out of bounds otherwise; here any n_i >= 0, n_j >= 0 is valid.
+ * tune may be NULL.  i- and j-sets may alias (same buffers). */
+int khrr_accel(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int n_j,
+               const float* pos_j, const float* m_j, const float* h_j, float* acc, float* phi,
+               unsigned flags, const khrr_tune* tune);
+
+/* Batched form: n_sys independent systems stored back to back.  sys_off is
+ * a DEVICE int array of n_sys+1 prefix offsets (system s owns particles
+ * [sys_off[s], sys_off[s+1])); every system interacts only with itself.
+ * pos/acc are vec3 planes of n_total = sys_off[n_sys] entries.  The work
+ * list (system, i-block, j-range) is built on the host from host_sys_off
+ * (a HOST copy of the same offsets) in longest-first order and all systems
+ * run in one persistent launch. */
+int khrr_accel_batched(khrr_ctx* ctx, int n_sys, const int* sys_off, const int* host_sys_off,
+                       const float* pos, const float* h, const float* m, float* acc, float* phi,
+                       unsigned flags, const khrr_tune* tune);
+
+/* Project-written autotuner: times candidate launch shapes of khrr_accel
+ * on synthetic bodies of size n on this context's device (synchronous) and
+ * returns the fastest.  Results are cached per power-of-two size bucket in
+ * the context; khrr_accel with tune == NULL uses the cache (or a static
+ * heuristic when the bucket was never tuned). */
+int khrr_tune_gravity(khrr_ctx* ctx, int n, khrr_tune* best, double* best_ms);
+/* Same for the batched form on the composition host_sys_off[0..n_sys]
+ * (i_per_lane, waves_per_block; j_split is always 1 there).  The result is
+ * what khrr_accel_batched uses when its tune argument is NULL. */
+int khrr_tune_batched(khrr_ctx* ctx, int n_sys, const int* host_sys_off, khrr_tune* best,
+                      double* best_ms);
+/* Static choice the library makes for n when nothing is tuned. */
+int khrr_tune_default(khrr_ctx* ctx, int n, khrr_tune* out);
+
+/* ---- integrator (a3) ------------------------------------------------- */
+/* DataBlock fields used by dev_fex (common.cuh:40-56) */
+typedef struct {
+  double Mh, Mh_inf, a, Rh, Mb, b, Rb, Rh2, con, const1, c_phi_h, c_phi_b;
+} khrr_extpot;
+/* Host-side constants exactly as nbody.cu:1111-1120 (Mh = Mb = 0 gives an
+ * identically zero external force). */
+int khrr_extpot_init(double Mh, double a, double Rh, double Mb, double b, double Rb,
+                     khrr_extpot* out);
+/* Host evaluation of dev_fex(p) (nbody-kernel.cuh:8-40), for tests. */
+int khrr_extpot_force(const khrr_extpot* ext, const double p[3], double f[3]);
+
+/* kernelNbody_integTime (nbody-kernel.cuh:144-194), FP64 state, f = acc +
+ * dev_fex(pos):
+ *   istep 0 (predictor): vel_t = vel + dt f;  pos_t = pos + dt/2 (vel + vel_t)
+ *   istep 1 (corrector): vel_t = (vel_t + vel + dt f)/2;  pos_t = pos;
+ *                        acc0 = acc
+ * pos_t, vel_t, pos, vel: vec3 planes of n doubles; acc, acc0: vec3 planes
+ * of n floats.  pos32 (optional, vec3 planes of n floats) receives
+ * (float)pos_t for the next gravity call.  ext may be NULL (no external
+ * potential).  t is accepted for signature parity; dev_fex ignores it. */
+int khrr_integrate(khrr_ctx* ctx, int n, int istep, double dt, double t, const khrr_extpot* ext,
+                   const float* acc, double* pos_t, double* vel_t, const double* pos,
+                   const double* vel, float* acc0, float* pos32);
+
+/* ---- diagnostics (a9) ------------------------------------------------ */
+#define KHRR_DIAG_EK 0     /* sum m v^2        (nbody.cu:824)               */
+#define KHRR_DIAG_EP 1     /* sum m phi        (nbody.cu:825)               */
+#define KHRR_DIAG_PX 2     /* momentum         (nbody.cu:821-823)           */
+#define KHRR_DIAG_PY 3
+#define KHRR_DIAG_PZ 4
+#define KHRR_DIAG_LX 5     /* angular momentum (nbody.cu:818-820)           */
+#define KHRR_DIAG_LY 6
+#define KHRR_DIAG_LZ 7
+#define KHRR_DIAG_M 8      /* total mass                                    */
+#define KHRR_DIAG_MX 9     /* sum m x (centre of mass * M)                  */
+#define KHRR_DIAG_MY 10
+#define KHRR_DIAG_MZ 11
+#define KHRR_DIAG_VR_MAX 12  /* max |v_r|  (nbody.cu:827-831)               */
+#define KHRR_DIAG_VFI_MAX 13 /* max v_phi (signed, starts at 0)             */
+#define KHRR_DIAG_VZ_MAX 14
+#define KHRR_DIAG_R_MAX 15
+#define KHRR_DIAG_Z_MAX 16
+#define KHRR_DIAG_COUNT 17
+/* FP64 reductions on the device over n particles (deterministic fixed-order
+ * tree).  pos, vel: vec3 planes of n doubles; mass: n doubles; phi: n
+ * floats (may be NULL: EP = 0).  out: HOST array of KHRR_DIAG_COUNT
+ * doubles.  Synchronous. */
+int khrr_diag(khrr_ctx* ctx, int n, const double* pos, const double* vel, const double* mass,
+              const float* phi, double* out);
+
+/* ---- p2mesh binning (a5-a7) ------------------------------------------ */
+typedef struct {
+  double min[3];    /* domainMin  (WaveEqData, common.cuh:58)            */
+  double cell[3];   /* cellSize                                           */
+  int size[3];      /* gridSize;  cells = size[0]*size[1]*size[2]         */
+} khrr_grid;
+#define KHRR_BIN_BLOCK 512 /* BLOCK_SIZE of the reference's cell blocks   */
+/* Bins n particles (pos: vec3 planes of n doubles, mass: n doubles).
+ *   cell_id[i]     = iz*sx*sy + iy*sz + ix with i* = clamp((int)((p-min)/cell),
+ *                    0, size-1) in FP64, saturating conversion, NaN -> 0, z
+ *                    clamped with size[1]   (p2mesh-kernel.cuh:25-43)
+ *   id_in_cell[i]  = rank of i among the particles of its cell in index order
+ *                    (the reference's atomic order is unspecified, :49)
+ *   count[c]       = particles in cell c                      (:55)
+ *   offset[c]      = exclusive prefix sum of count (per cell)
+ *   start_id_ref[c]= sum of counts of all 512-cell blocks before block(c),
+ *                    the value the reference's CellInfo.start_id holds after
+ *                    computePrefixSums + adjustGlobalPrefixSums (:62-123)
+ *   block_incl[b]  = inclusive prefix of 512-cell block totals (maxPBC)
+ *   cell_mass[c]   = FP64 sum of mass over the cell in index order (:126-137)
+ * Any output except count may be NULL.  Scratch is owned by the context. */
+int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos,
+                    const double* mass, int* cell_id, int* id_in_cell, int* count, int* offset,
+                    int* start_id_ref, int* block_incl, double* cell_mass);
+
+/* computeCellPhiUnsorted (p2mesh-kernel.cuh:152-167): for every particle i
+ * of cell c, cell_phi[c] += phi[i] / count[c], summed in FP64 in particle
+ * index order (the reference's FP64 atomics have no fixed order).  Uses
+ * the cell_id, id_in_cell, count and offset arrays of khrr_p2mesh_bin.
+ * accumulate == 0 starts every cell from 0; accumulate != 0 adds to the
+ * existing cell_phi, which is what the reference's save path does with a
+ * buffer it never clears (nbody.cu:1628-1634).  cell_phi: G doubles. */
+int khrr_p2mesh_cell_phi(khrr_ctx* ctx, int n, long long n_cells, const int* cell_id,
+                         const int* id_in_cell, const int* count, const int* offset,
+                         const float* phi, double* cell_phi, int accumulate);
+
+/* ---- damped-wave Poisson stencil (a8) -------------------------------- */
+typedef struct {
+  int nx;            /* cells per edge (cubic grid, AT() index x fastest) */
+  double dx;         /* dx_wave   */
+  double c;          /* c_wave    */
+  double dt;         /* dt_wave   */
+  double diss_base;
+  double diss_extra;
+  double sim_l;
+  double bc_l;
+} khrr_wave;
+/* src[c] = (float)(4*PI*G*cell_mass[c]/dx^3), G = 1 (wave-kernel.cuh:36-38) */
+int khrr_wave_source(khrr_ctx* ctx, const khrr_wave* w, const double* cell_mass, float* src);
+/* One wave_diss_iteration: phi_new from phi, phi_old, src (FP32, nx^3). */
+int khrr_wave_step(khrr_ctx* ctx, const khrr_wave* w, float* phi_new, const float* phi,
+                   const float* phi_old, const float* src);
+/* `iters` iterations with the reference's buffer rotation (nbody.cu:227-239):
+ *   next = step(curr, prev); prev <- curr; curr <- next; next <- old prev.
+ * The three pointers are updated in place (host pointers to device pointers),
+ * so afterwards *curr holds the newest field and *prev the one before. */
+int khrr_wave_iterate(khrr_ctx* ctx, const khrr_wave* w, float** prev, float** curr,
+                      float** next, const float* src, int iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KHRR_H */

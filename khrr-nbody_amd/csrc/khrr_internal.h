@@ -1,0 +1,132 @@
+// khrr_internal.h -- shared plumbing of the gfx950 library: the context
+// object (stream, error text, scratch arena, kernel timers, tuning cache)
+// and small helpers used by every kernel file.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "khrr.h"
+
+namespace khrr {
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+struct TimerPair {
+  hipEvent_t start;
+  hipEvent_t stop;
+  int cls;
+};
+
+}  // namespace khrr
+
+struct khrr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int cu_count = 256;
+  // Scratch arena: one device allocation grown on demand; callers carve
+  // stream-ordered sub-buffers out of it for the duration of one call.
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // Small persistent device words (flags, counters): 256 B.
+  int* dev_words = nullptr;
+  // Pinned host staging for synchronous results.
+  double* host_stage = nullptr;
+  // Kernel timing.
+  bool timing = false;
+  std::vector<khrr::TimerPair> pending;
+  std::vector<hipEvent_t> event_pool;
+  long launches[KHRR_K_COUNT] = {0};
+  double total_ms[KHRR_K_COUNT] = {0};
+  // Autotuner cache, indexed by ceil(log2(n)).
+  khrr_tune tuned[40] = {};
+  bool has_tuned[40] = {};
+  khrr_tune batch_tuned = {};
+  bool has_batch_tuned = false;
+  // Batched form: the work list (system, i-group) of the last composition,
+  // kept on the device until the composition or the i-group width changes.
+  std::vector<int> batch_off;
+  int batch_ipl = 0;
+  int batch_items = 0;
+  void* batch_dev = nullptr;  // items (int2) then per-system uniform flags
+  size_t batch_dev_bytes = 0;
+  void* batch_pinned = nullptr;
+  size_t batch_pinned_bytes = 0;
+};
+
+namespace khrr {
+
+// Records an error on ctx (or the thread-local slot when ctx is null) and
+// returns `code`.
+int fail(khrr_ctx* ctx, int code, const char* fmt, ...);
+// Thread-local message for calls without a context.
+std::string& thread_error();
+
+// Makes the context's device current for the calling host thread.
+int enter(khrr_ctx* ctx);
+
+// Ensures the scratch arena holds at least `bytes`; returns its base.
+int scratch(khrr_ctx* ctx, size_t bytes, void** base);
+
+// Bump allocator over a scratch region (256-byte aligned pieces).
+struct Carver {
+  char* base;
+  size_t used = 0;
+  explicit Carver(void* b) : base(static_cast<char*>(b)) {}
+  template <typename T>
+  T* take(size_t count) {
+    T* p = reinterpret_cast<T*>(base + used);
+    used += (count * sizeof(T) + 255) & ~size_t(255);
+    return p;
+  }
+  static size_t need(size_t count, size_t elem) { return (count * elem + 255) & ~size_t(255); }
+};
+
+// RAII bracket that records start/stop events around the launches made in
+// its scope when the context has timing enabled.
+class Timed {
+ public:
+  Timed(khrr_ctx* ctx, int cls);
+  ~Timed();
+
+ private:
+  khrr_ctx* ctx_;
+  int cls_;
+  hipEvent_t start_ = nullptr;
+};
+
+// Checks the launch that was just made.
+int check_launch(khrr_ctx* ctx, const char* what);
+
+inline unsigned ceil_div(long long a, long long b) { return unsigned((a + b - 1) / b); }
+
+// Gravity launch helpers shared by the plain entry point, the batched form
+// and the autotuner (khrr_gravity.hip).
+int validate_tune(khrr_ctx* ctx, const khrr_tune& t);
+khrr_tune default_tune(const khrr_ctx* ctx, int n_i, int n_j);
+const khrr_tune* cached_tune(const khrr_ctx* ctx, int n);
+void store_tune(khrr_ctx* ctx, int n, const khrr_tune& t);
+int accel_launch(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int n_j,
+                 const float* pos_j, const float* m_j, const float* h_j, float* acc, float* phi,
+                 unsigned flags, const khrr_tune& tune);
+
+}  // namespace khrr
+
+#define KHRR_TRY(expr)       \
+  do {                       \
+    int rc_ = (expr);        \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+#define KHRR_HIP(ctx, call)                                                              \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return khrr::fail((ctx), KHRR_EDEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)

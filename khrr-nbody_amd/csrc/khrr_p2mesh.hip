@@ -1,0 +1,367 @@
+// khrr_p2mesh.hip -- deterministic particle-to-mesh binning (gfx950).
+//
+// Replaces the sequence convert_particles_to_grid (nbody/src/nbody.cu:139-181):
+//   assignParticlesToCells     p2mesh-kernel.cuh:11-57
+//   computePrefixSums          p2mesh-kernel.cuh:62-105
+//   adjustGlobalPrefixSums     p2mesh-kernel.cuh:110-123
+//   computeCellMassesUnsorted  p2mesh-kernel.cuh:126-137
+//
+// Pipeline (all integer/FP64, bit-exact with a sequential evaluation):
+//  1. assign:  cell index per particle from FP64 positions, same arithmetic
+//              as the reference (subtract, IEEE divide, truncate with GPU
+//              cvt saturation, clamp; z clamped with size[1]; y stride sz).
+//  2. sort:    stable LSD radix sort of (cell, particle index), 8-bit
+//              digits; in-block ranks come from 64-lane ballot matching, so
+//              equal cells keep index order.
+//  3. cells:   one 512-thread block per 512-cell block of the reference.
+//              Two binary searches on the sorted cells give the number of
+//              particles before the block (= the reference's block-granular
+//              start_id) and through it (= maxPBC); an LDS histogram and
+//              scan give per-cell counts and exact exclusive offsets.  Each
+//              non-empty cell walks its sorted segment in index order to
+//              write id_in_cell and the FP64 mass sum.
+// Every cell is written exactly once, so no output needs clearing first.
+#include <climits>
+
+#include "khrr_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kSortThreads = 256;  // == kRadix: one digit per thread
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kCellBlock = KHRR_BIN_BLOCK;  // 512 cells, BLOCK_SIZE of common.cuh:8
+
+struct GridArgs {
+  double min0, min1, min2;
+  double cell0, cell1, cell2;
+  int s0, s1, s2;
+};
+
+// (int) conversion as GPU cvt does it: truncate toward zero, saturate to
+// the int range, NaN -> 0.
+__device__ __forceinline__ int sat_trunc(double v) {
+  if (v != v) return 0;
+  if (v >= 2147483648.0) return INT_MAX;
+  if (v <= -2147483649.0) return INT_MIN;
+  return (int)v;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+__global__ __launch_bounds__(256) void bin_assign(int n, const double* __restrict__ pos,
+                                                  GridArgs g, int* __restrict__ cell) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t N = (size_t)n;
+  const int ix = clampi(sat_trunc((pos[i] - g.min0) / g.cell0), 0, g.s0 - 1);
+  const int iy = clampi(sat_trunc((pos[N + i] - g.min1) / g.cell1), 0, g.s1 - 1);
+  const int iz = clampi(sat_trunc((pos[2 * N + i] - g.min2) / g.cell2), 0, g.s1 - 1);
+  cell[i] = iz * g.s0 * g.s1 + iy * g.s2 + ix;
+}
+
+// Per-tile digit histogram, written digit-major: hist[d * ntiles + tile].
+__global__ __launch_bounds__(kSortThreads) void radix_hist(int n, const int* __restrict__ keys,
+                                                           int shift, int ntiles,
+                                                           int* __restrict__ hist) {
+  __shared__ int h[kRadix];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * kSortTile;
+  for (int it = 0; it < kSortItems; ++it) {
+    const int idx = base + it * kSortThreads + threadIdx.x;
+    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kRadix - 1)], 1);
+  }
+  __syncthreads();
+  hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// In-place exclusive scan of len ints by one 1024-thread block.
+__global__ __launch_bounds__(1024) void scan_exclusive(int len, int* __restrict__ data) {
+  __shared__ int sums[1024];
+  const int per = (len + 1023) / 1024;
+  const int t0 = threadIdx.x * per;
+  const int t1 = min(len, t0 + per);
+  int s = 0;
+  for (int k = t0; k < t1; ++k) s += data[k];
+  sums[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = threadIdx.x >= (unsigned)off ? sums[threadIdx.x - off] : 0;
+    __syncthreads();
+    sums[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = threadIdx.x > 0 ? sums[threadIdx.x - 1] : 0;
+  for (int k = t0; k < t1; ++k) {
+    const int v = data[k];
+    data[k] = run;
+    run += v;
+  }
+}
+
+// Stable scatter of one digit pass.  Items are visited in index order
+// (round-major, then wave, then lane); a lane's rank among equal digits of
+// its wave comes from eight ballots, and the LDS counters carry the running
+// per-digit position across waves and rounds.
+__global__ __launch_bounds__(kSortThreads) void radix_scatter(
+    int n, const int* __restrict__ keys_in, const int* __restrict__ vals_in, int shift,
+    int ntiles, const int* __restrict__ offs, int* __restrict__ keys_out,
+    int* __restrict__ vals_out) {
+  constexpr int kWaves = kSortThreads / khrr::kWave;
+  __shared__ int run[kRadix];
+  __shared__ int wcnt[kWaves][kRadix];
+  __shared__ int wpre[kWaves][kRadix];
+  const int tid = threadIdx.x;
+  const int lane = tid & (khrr::kWave - 1);
+  const int wave = tid / khrr::kWave;
+  run[tid] = offs[tid * ntiles + blockIdx.x];
+  for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
+  __syncthreads();
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const int base = blockIdx.x * kSortTile;
+  for (int it = 0; it < kSortItems; ++it) {
+    const int idx = base + it * kSortThreads + tid;
+    const bool valid = idx < n;
+    const int key = valid ? keys_in[idx] : 0;
+    const int val = valid ? (vals_in ? vals_in[idx] : idx) : 0;
+    const int d = (key >> shift) & (kRadix - 1);
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kRadixBits; ++b) {
+      const bool bit = (d >> b) & 1;
+      const unsigned long long bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    const int rank = __popcll(peers & lt);
+    if (valid && (peers & lt) == 0ull) wcnt[wave][d] = __popcll(peers);
+    __syncthreads();
+    {
+      int b = run[tid];
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) {
+        wpre[w][tid] = b;
+        b += wcnt[w][tid];
+        wcnt[w][tid] = 0;
+      }
+      run[tid] = b;
+    }
+    __syncthreads();
+    if (valid) {
+      const int p = wpre[wave][d] + rank;
+      keys_out[p] = key;
+      vals_out[p] = val;
+    }
+  }
+}
+
+__device__ __forceinline__ int lower_bound_dev(const int* __restrict__ a, int n, long long key) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = lo + ((hi - lo) >> 1);
+    if ((long long)a[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kCellBlock) void bin_cells(
+    int n, long long G, const int* __restrict__ skeys, const int* __restrict__ svals,
+    const double* __restrict__ mass, int* __restrict__ count, int* __restrict__ offset,
+    int* __restrict__ start_ref, int* __restrict__ block_incl, int* __restrict__ id_in_cell,
+    double* __restrict__ cell_mass) {
+  __shared__ int cnt[kCellBlock];
+  __shared__ int wsum[kCellBlock / khrr::kWave];
+  __shared__ int bounds[2];
+  const int tid = threadIdx.x;
+  const long long c0 = (long long)blockIdx.x * kCellBlock;
+  if (tid == 0) bounds[0] = lower_bound_dev(skeys, n, c0);
+  if (tid == khrr::kWave) bounds[1] = lower_bound_dev(skeys, n, c0 + kCellBlock);
+  cnt[tid] = 0;
+  __syncthreads();
+  const int lo = bounds[0], hi = bounds[1];
+  for (int p = lo + tid; p < hi; p += kCellBlock) atomicAdd(&cnt[skeys[p] - (int)c0], 1);
+  __syncthreads();
+  // Exclusive scan of cnt: wave-inclusive scans, then wave totals.
+  const int lane = tid & (khrr::kWave - 1);
+  const int wave = tid / khrr::kWave;
+  const int mine = cnt[tid];
+  int incl = mine;
+#pragma unroll
+  for (int off = 1; off < khrr::kWave; off <<= 1) {
+    const int v = __shfl_up(incl, off, khrr::kWave);
+    if (lane >= off) incl += v;
+  }
+  if (lane == khrr::kWave - 1) wsum[wave] = incl;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += wsum[w];
+  const int excl = lo + before + incl - mine;
+  const long long c = c0 + tid;
+  if (c < G) {
+    count[c] = mine;
+    if (offset) offset[c] = excl;
+    if (start_ref) start_ref[c] = lo;
+    double m = 0.0;
+    for (int k = 0; k < mine; ++k) {
+      const int v = svals[excl + k];
+      if (id_in_cell) id_in_cell[v] = k;
+      if (cell_mass) m += mass[v];
+    }
+    if (cell_mass) cell_mass[c] = m;
+  }
+  if (tid == 0 && block_incl) block_incl[blockIdx.x] = hi;
+}
+
+// perm[offset[cell_id[i]] + id_in_cell[i]] = i: the sorted segments back
+// from the binning outputs.  Entries that do not come from a consistent
+// binning (index outside the grid or the particle range) are skipped.
+__global__ __launch_bounds__(256) void segments_from_bins(int n, long long G,
+                                                          const int* __restrict__ cell_id,
+                                                          const int* __restrict__ id_in_cell,
+                                                          const int* __restrict__ offset,
+                                                          int* __restrict__ perm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell_id[i];
+  if (c < 0 || c >= G) return;
+  const long long p = (long long)offset[c] + id_in_cell[i];
+  if (p >= 0 && p < n) perm[p] = i;
+}
+
+// computeCellPhiUnsorted (p2mesh-kernel.cuh:152-167) in index order.
+__global__ __launch_bounds__(256) void cell_phi_kernel(int n, long long G,
+                                                       const int* __restrict__ count,
+                                                       const int* __restrict__ offset,
+                                                       const int* __restrict__ perm,
+                                                       const float* __restrict__ phi,
+                                                       double* __restrict__ cell_phi,
+                                                       int accumulate) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G) return;
+  const int k = count[c];
+  double v = accumulate ? cell_phi[c] : 0.0;
+  const long long o = k > 0 ? offset[c] : 0;
+  for (int q = 0; q < k; ++q) {
+    if (o + q < 0 || o + q >= n) break;
+    const int i = perm[o + q];
+    if (i >= 0 && i < n) v += (double)phi[i] / k;
+  }
+  cell_phi[c] = v;
+}
+
+}  // namespace
+
+extern "C" int khrr_p2mesh_cell_phi(khrr_ctx* ctx, int n, long long n_cells, const int* cell_id,
+                                    const int* id_in_cell, const int* count, const int* offset,
+                                    const float* phi, double* cell_phi, int accumulate) {
+  using namespace khrr;
+  KHRR_TRY(enter(ctx));
+  if (n < 0 || n_cells < 1 || n_cells > INT_MAX)
+    return fail(ctx, KHRR_EINVAL, "bad sizes n=%d cells=%lld", n, n_cells);
+  if (!count || !offset || !cell_phi || (n > 0 && (!cell_id || !id_in_cell || !phi)))
+    return fail(ctx, KHRR_EINVAL, "null pointer");
+  void* base = nullptr;
+  KHRR_TRY(scratch(ctx, Carver::need((size_t)(n > 0 ? n : 1), sizeof(int)), &base));
+  int* perm = static_cast<int*>(base);
+  Timed tm(ctx, KHRR_K_BIN);
+  if (n > 0)
+    hipLaunchKernelGGL(segments_from_bins, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, n,
+                       n_cells, cell_id, id_in_cell, offset, perm);
+  hipLaunchKernelGGL(cell_phi_kernel, dim3(ceil_div(n_cells, 256)), dim3(256), 0, ctx->stream, n,
+                     n_cells, count, offset, perm, phi, cell_phi, accumulate);
+  return check_launch(ctx, "cell_phi kernels");
+}
+
+extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos,
+                               const double* mass, int* cell_id, int* id_in_cell, int* count,
+                               int* offset, int* start_id_ref, int* block_incl,
+                               double* cell_mass) {
+  using namespace khrr;
+  KHRR_TRY(enter(ctx));
+  if (!grid) return fail(ctx, KHRR_EINVAL, "grid is NULL");
+  if (n < 0) return fail(ctx, KHRR_EINVAL, "n < 0");
+  const long long G = (long long)grid->size[0] * grid->size[1] * grid->size[2];
+  if (grid->size[0] <= 0 || grid->size[1] <= 0 || grid->size[2] <= 0 || G > INT_MAX)
+    return fail(ctx, KHRR_EINVAL, "invalid grid size %d x %d x %d", grid->size[0], grid->size[1],
+                grid->size[2]);
+  // The reference clamps z with size[1] and strides y by size[2]; for a
+  // non-cubic grid that can index past G, which this entry point rejects.
+  const long long max_cell = (long long)(grid->size[1] - 1) * grid->size[0] * grid->size[1] +
+                             (long long)(grid->size[1] - 1) * grid->size[2] + (grid->size[0] - 1);
+  if (max_cell >= G)
+    return fail(ctx, KHRR_EINVAL,
+                "grid %d x %d x %d: the reference's cell index (z clamped by size[1], y "
+                "stride size[2]) can exceed the cell count for this shape",
+                grid->size[0], grid->size[1], grid->size[2]);
+  if (!count) return fail(ctx, KHRR_EINVAL, "count is NULL");
+  if (n > 0 && !pos) return fail(ctx, KHRR_EINVAL, "pos is NULL");
+  if (n > 0 && cell_mass && !mass) return fail(ctx, KHRR_EINVAL, "mass is NULL");
+
+  const int ntiles = n > 0 ? (int)ceil_div(n, kSortTile) : 1;
+  const size_t need = 5 * Carver::need((size_t)(n > 0 ? n : 1), sizeof(int)) +
+                      Carver::need((size_t)kRadix * ntiles, sizeof(int));
+  void* base = nullptr;
+  KHRR_TRY(scratch(ctx, need, &base));
+  Carver cv(base);
+  int* cells = cv.take<int>(n > 0 ? n : 1);
+  int* ka = cv.take<int>(n > 0 ? n : 1);
+  int* va = cv.take<int>(n > 0 ? n : 1);
+  int* kb = cv.take<int>(n > 0 ? n : 1);
+  int* vb = cv.take<int>(n > 0 ? n : 1);
+  int* hist = cv.take<int>((size_t)kRadix * ntiles);
+  if (cell_id) cells = cell_id;
+
+  GridArgs g{grid->min[0], grid->min[1], grid->min[2], grid->cell[0], grid->cell[1],
+             grid->cell[2], grid->size[0], grid->size[1], grid->size[2]};
+  int bits = 0;
+  while (bits < 31 && (1ll << bits) < G) ++bits;
+  const int passes = (bits + kRadixBits - 1) / kRadixBits;
+
+  Timed tm(ctx, KHRR_K_BIN);
+  const int* skeys = cells;
+  const int* svals = nullptr;
+  if (n > 0) {
+    hipLaunchKernelGGL(bin_assign, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, n, pos, g,
+                       cells);
+    const int* kin = cells;
+    const int* vin = nullptr;
+    for (int p = 0; p < passes; ++p) {
+      int* kout = (p & 1) ? kb : ka;
+      int* vout = (p & 1) ? vb : va;
+      hipLaunchKernelGGL(radix_hist, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, kin,
+                         p * kRadixBits, ntiles, hist);
+      hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(1024), 0, ctx->stream, kRadix * ntiles,
+                         hist);
+      hipLaunchKernelGGL(radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, kin,
+                         vin, p * kRadixBits, ntiles, hist, kout, vout);
+      kin = kout;
+      vin = vout;
+    }
+    skeys = kin;
+    svals = vin;
+    if (passes == 0) {
+      // A single cell: the identity permutation is already sorted.
+      hipLaunchKernelGGL(radix_hist, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, cells,
+                         0, ntiles, hist);
+      hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(1024), 0, ctx->stream, kRadix * ntiles,
+                         hist);
+      hipLaunchKernelGGL(radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n,
+                         cells, nullptr, 0, ntiles, hist, ka, va);
+      skeys = ka;
+      svals = va;
+    }
+  }
+  const unsigned cblocks = ceil_div(G, kCellBlock);
+  hipLaunchKernelGGL(bin_cells, dim3(cblocks), dim3(kCellBlock), 0, ctx->stream, n, G, skeys,
+                     svals, mass, count, offset, start_id_ref, block_incl, id_in_cell, cell_mass);
+  return check_launch(ctx, "p2mesh kernels");
+}

@@ -1,0 +1,121 @@
+"""The batched form (khrr_accel_batched): many independent systems of uneven N
+in one persistent launch.  Each system's result must equal the plain
+single-system launch (j_split 1, j_tile 256) bit for bit, whatever else is
+in the batch, and must match the FP64 oracle within the gravity tolerance
+(median <= 1e-6, p99 <= 1e-5)."""
+import numpy as np
+import pytest
+
+import ics
+import oracle as orc
+from helpers import err_stats, rel_err, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _population(sizes, seed0=100000):
+    parts = []
+    for k, n in enumerate(sizes):
+        pos, vel = ics.plummer(int(n), 1.0, 0.3333, 1.0, seed=seed0 + k)
+        eps = 0.004 * (n / 16384.0) ** (-1.0 / 3.0)
+        parts.append((pos, np.full(n, 1.0 / n), np.full(n, 0.5 * eps * eps)))
+    off = np.concatenate([[0], np.cumsum([p[0].shape[1] for p in parts])]).astype(np.int32)
+    pos = np.concatenate([p[0] for p in parts], axis=1).astype(np.float32)
+    m = np.concatenate([p[1] for p in parts]).astype(np.float32)
+    h = np.concatenate([p[2] for p in parts]).astype(np.float32)
+    return off, pos, m, h
+
+
+def _run_batched(ctx, off, pos, m, h, tune=None, phi=False, compensated=False):
+    import torch
+    n = pos.shape[1]
+    acc = torch.zeros((3, n), device="cuda:0")
+    ph = torch.zeros(n, device="cuda:0") if phi else None
+    ctx.accel_batched(to_dev(off, torch.int32), off.tolist(), to_dev(pos, torch.float32),
+                      to_dev(h, torch.float32), to_dev(m, torch.float32), acc=acc, phi=ph,
+                      tune=tune, compensated=compensated)
+    torch.cuda.synchronize()
+    return acc.cpu().numpy(), (ph.cpu().numpy() if phi else None)
+
+
+def _run_plain(ctx, pos, m, h, phi=False, compensated=False):
+    import khrr
+    import torch
+    n = pos.shape[1]
+    acc = torch.zeros((3, n), device="cuda:0")
+    ph = torch.zeros(n, device="cuda:0") if phi else None
+    p = to_dev(pos, torch.float32)
+    hh = to_dev(h, torch.float32)
+    ctx.accel(p, hh, p, to_dev(m, torch.float32), hh, acc=acc, phi=ph,
+              tune=khrr.Tune(4, 4, 1, 256), compensated=compensated)
+    torch.cuda.synchronize()
+    return acc.cpu().numpy(), (ph.cpu().numpy() if phi else None)
+
+
+def test_batched_equals_plain_per_system(ctx):
+    sizes = [1024, 3072, 7168, 2048, 1024, 5120, 64, 1]
+    off, pos, m, h = _population(sizes)
+    acc, phi = _run_batched(ctx, off, pos, m, h, phi=True)
+    for s in range(len(sizes)):
+        a, b = off[s], off[s + 1]
+        pa, pp = _run_plain(ctx, pos[:, a:b], m[a:b], h[a:b], phi=True)
+        np.testing.assert_array_equal(acc[:, a:b], pa, err_msg=f"system {s}")
+        np.testing.assert_array_equal(phi[a:b], pp, err_msg=f"system {s}")
+
+
+@pytest.mark.parametrize("ipl", [2, 4, 8])
+def test_batched_oracle_and_shapes(ctx, ipl):
+    import khrr
+    sizes = [4096, 1024, 2048, 3072]
+    off, pos, m, h = _population(sizes, seed0=200)
+    acc, _ = _run_batched(ctx, off, pos, m, h, tune=khrr.Tune(ipl, 2, 1, 256))
+    for s in range(len(sizes)):
+        a, b = off[s], off[s + 1]
+        ref, _ = orc.accel(pos[:, a:b], h[a:b], pos[:, a:b], m[a:b], h[a:b], want_phi=False)
+        st = err_stats(rel_err(acc[:, a:b], ref))
+        assert st["median"] <= 1e-6 and st["p99"] <= 1e-5, (s, st)
+
+
+def test_alone_equals_in_the_mix(ctx):
+    # the same system alone and embedded among others (different position
+    # in the work list, different neighbours): identical bits
+    off, pos, m, h = _population([2048, 5120, 1024], seed0=300)
+    acc_mix, _ = _run_batched(ctx, off, pos, m, h)
+    a, b = off[1], off[2]
+    solo_off = np.array([0, b - a], np.int32)
+    acc_solo, _ = _run_batched(ctx, solo_off, pos[:, a:b].copy(), m[a:b].copy(), h[a:b].copy())
+    np.testing.assert_array_equal(acc_mix[:, a:b], acc_solo)
+
+
+def test_composition_change_and_empty_systems(ctx):
+    off, pos, m, h = _population([1024, 0, 2048], seed0=400)
+    acc1, _ = _run_batched(ctx, off, pos, m, h)
+    off2, pos2, m2, h2 = _population([3072, 1024], seed0=500)
+    acc2, _ = _run_batched(ctx, off2, pos2, m2, h2)
+    acc1b, _ = _run_batched(ctx, off, pos, m, h)
+    np.testing.assert_array_equal(acc1, acc1b)
+    assert np.isfinite(acc2).all()
+
+
+def test_bad_offsets_rejected(ctx):
+    import khrr
+    import torch
+    pos = torch.zeros((3, 4), device="cuda:0")
+    h = torch.zeros(4, device="cuda:0")
+    with pytest.raises(khrr.KhrrError, match="non-decreasing"):
+        ctx.accel_batched(torch.tensor([0, 3, 2, 4], dtype=torch.int32, device="cuda:0"),
+                          [0, 3, 2, 4], pos, h, h, acc=torch.zeros((3, 4), device="cuda:0"))
+
+
+def test_autotuners_return_valid_shapes(ctx):
+    t, ms = ctx.tune_gravity(20000)
+    assert t.i_per_lane in (1, 2, 4, 8) and 1 <= t.waves_per_block <= 4 and ms > 0
+    off, pos, m, h = _population([1024, 2048, 4096], seed0=600)
+    tb, msb = ctx.tune_batched(off.tolist())
+    assert tb.i_per_lane in (2, 4, 8) and tb.j_split == 1 and msb > 0
+    # the tuned shape is what a NULL tune now uses; results stay exact
+    acc, _ = _run_batched(ctx, off, pos, m, h)
+    for s in range(3):
+        a, b = off[s], off[s + 1]
+        pa, _ = _run_plain(ctx, pos[:, a:b], m[a:b], h[a:b])
+        np.testing.assert_array_equal(acc[:, a:b], pa)

@@ -1,0 +1,110 @@
+"""GPU parity of khrr_p2mesh_bin (p2mesh-kernel.cuh:11-137, the sequence of
+nbody.cu:139-181) against the oracle: every output is compared bit for bit
+(integer outputs and the index-order FP64 cell masses)."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from helpers import galaxy_system, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(nx, lo, hi):
+    import khrr
+    return khrr.cube_grid(nx, lo, hi)
+
+
+def _bin_gpu(ctx, grid, pos, mass):
+    import torch
+    n = pos.shape[1]
+    G = grid.size[0] * grid.size[1] * grid.size[2]
+    B = (G + 511) // 512
+    i32 = dict(device="cuda:0", dtype=torch.int32)
+    out = dict(cell_id=torch.full((n,), -7, **i32), id_in_cell=torch.full((n,), -7, **i32),
+               count=torch.full((G,), -7, **i32), offset=torch.full((G,), -7, **i32),
+               start_id_ref=torch.full((G,), -7, **i32), block_incl=torch.full((B,), -7, **i32),
+               cell_mass=torch.full((G,), -7.0, device="cuda:0", dtype=torch.float64))
+    ctx.p2mesh_bin(grid, to_dev(pos, torch.float64), to_dev(mass, torch.float64), **out)
+    torch.cuda.synchronize()
+    return {k: v.cpu().numpy() for k, v in out.items()}
+
+
+def _bin_ref(grid, pos, mass):
+    return orc.p2mesh_bin(list(grid.min), list(grid.cell), list(grid.size), pos, mass)
+
+
+def _check_equal(got, ref):
+    for k in ref:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize("nx,lo,hi,n", [(64, -2.0, 2.0, 20000), (13, -1.5, 1.5, 3001),
+                                        (200, -20.0, 20.0, 5000), (1, -1.0, 1.0, 100)])
+def test_bit_exact_vs_oracle(ctx, nx, lo, hi, n):
+    pos, _, mass, _ = galaxy_system(n // 3, n - n // 3, seed=nx)
+    g = _grid(nx, lo, hi)
+    _check_equal(_bin_gpu(ctx, g, pos, mass), _bin_ref(g, pos, mass))
+
+
+def test_full_size_headline_grid(ctx):
+    # 786432 bodies on 256^3 over [-2,2]^3 (the headline workload's binning)
+    pos, _, mass, _ = galaxy_system(262144, 524288, seed=12345)
+    g = _grid(256, -2.0, 2.0)
+    got = _bin_gpu(ctx, g, pos, mass)
+    ref = _bin_ref(g, pos, mass)
+    _check_equal(got, ref)
+    # size-independent properties
+    assert got["count"].sum() == pos.shape[1]
+    assert got["block_incl"][-1] == pos.shape[1]
+    assert abs(got["cell_mass"].sum() - mass.sum()) < 1e-9 * mass.sum()
+
+
+def test_edge_inputs(ctx):
+    # NaN -> 0 then clamp, +-inf and huge values saturate then clamp,
+    # everything outside lands on faces; many particles share one cell
+    n = 5000
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-3, 3, (3, n))
+    pos[:, :1000] = 0.01  # one dense cell
+    pos[0, 1000] = np.nan
+    pos[1, 1001] = np.inf
+    pos[2, 1002] = -np.inf
+    pos[:, 1003] = 1e300
+    pos[:, 1004] = -1e300
+    pos[0, 1005] = 2.0  # exactly the upper bound
+    mass = rng.uniform(0.1, 1.0, n)
+    g = _grid(16, -2.0, 2.0)
+    _check_equal(_bin_gpu(ctx, g, pos, mass), _bin_ref(g, pos, mass))
+
+
+def test_all_in_one_cell_and_empty(ctx):
+    g = _grid(8, -1.0, 1.0)
+    pos = np.full((3, 4000), 0.3)
+    mass = np.linspace(1.0, 2.0, 4000)
+    got = _bin_gpu(ctx, g, pos, mass)
+    _check_equal(got, _bin_ref(g, pos, mass))
+    np.testing.assert_array_equal(got["id_in_cell"], np.arange(4000))
+    empty = _bin_gpu(ctx, g, np.zeros((3, 0)), np.zeros(0))
+    assert np.all(empty["count"] == 0) and np.all(empty["offset"] == 0)
+    assert np.all(empty["cell_mass"] == 0.0)
+
+
+def test_reference_start_file_fixture(ctx):
+    import os
+    from conftest import GOLDEN
+    gd = np.load(os.path.join(GOLDEN, "p2mesh_golden.npz"))
+    pos = gd["pos"]
+    mass = np.full(pos.shape[1], 1.0 / 262144)
+    got200 = _bin_gpu(ctx, _grid(200, -20.0, 20.0), pos, mass)
+    np.testing.assert_array_equal(got200["cell_id"], gd["cell_200"])
+    got64 = _bin_gpu(ctx, _grid(64, -2.0, 2.0), pos, mass)
+    np.testing.assert_array_equal(got64["cell_id"], gd["cell_64"])
+
+
+def test_rejects_bad_grid(ctx):
+    import khrr
+    g = _grid(4, -1.0, 1.0)
+    g.size[0] = 0
+    with pytest.raises(khrr.KhrrError, match="invalid grid"):
+        _bin_gpu(ctx, g, np.zeros((3, 1)), np.ones(1))

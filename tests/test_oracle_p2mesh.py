@@ -1,0 +1,94 @@
+"""Pins the oracle's binning (p2mesh-kernel.cuh:11-137) with an independent
+numpy restatement, brute-force recounts and the golden cell indices."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import GOLDEN
+
+
+def _numpy_cells(pos, lo, d, s):
+    def axis(v, size):
+        q = (v - lo) / d
+        with np.errstate(invalid="ignore"):
+            k = np.where(np.isnan(q), 0, np.clip(q, -2.0 ** 31, 2.0 ** 31 - 1))
+        k = np.trunc(k).astype(np.int64)
+        return np.clip(k, 0, size - 1)
+    ix, iy, iz = axis(pos[0], s), axis(pos[1], s), axis(pos[2], s)
+    return (iz * s * s + iy * s + ix).astype(np.int32)
+
+
+def _random(n, seed, spread=2.5):
+    rng = np.random.default_rng(seed)
+    return rng.normal(scale=spread / 2, size=(3, n)), rng.uniform(0.1, 1.0, n)
+
+
+@pytest.mark.parametrize("s,lo,hi", [(16, -2.0, 2.0), (13, -1.0, 1.0), (64, -2.0, 2.0)])
+def test_against_numpy_recount(s, lo, hi):
+    pos, m = _random(20000, s)
+    d = (hi - lo) / s
+    out = orc.p2mesh_bin([lo] * 3, [d] * 3, [s] * 3, pos, m)
+    cells = _numpy_cells(pos, lo, d, s)
+    np.testing.assert_array_equal(out["cell_id"], cells)
+    G = s ** 3
+    count = np.bincount(cells, minlength=G)
+    np.testing.assert_array_equal(out["count"], count)
+    # exclusive per-cell scan
+    np.testing.assert_array_equal(out["offset"], np.concatenate([[0], np.cumsum(count)[:-1]]))
+    # block-granular offsets of the reference (computePrefixSums +
+    # adjustGlobalPrefixSums): sum of counts of all 512-blocks before
+    B = (G + 511) // 512
+    blk = np.array([count[b * 512:(b + 1) * 512].sum() for b in range(B)])
+    incl = np.cumsum(blk)
+    np.testing.assert_array_equal(out["block_incl"], incl)
+    excl = np.concatenate([[0], incl[:-1]])
+    np.testing.assert_array_equal(out["start_id_ref"], excl[np.arange(G) // 512])
+    # id_in_cell is the index-order rank: a permutation of 0..count-1
+    for c in np.unique(cells)[:200]:
+        members = np.nonzero(cells == c)[0]
+        np.testing.assert_array_equal(out["id_in_cell"][members], np.arange(len(members)))
+    # cell mass: index-order FP64 sum, and mass conservation
+    for c in np.unique(cells)[:200]:
+        acc = 0.0
+        for i in np.nonzero(cells == c)[0]:
+            acc += m[i]
+        assert out["cell_mass"][c] == acc
+    assert abs(out["cell_mass"].sum() - m.sum()) < 1e-12 * m.sum()
+
+
+def test_saturation_and_nan():
+    assert orc.sat_trunc(float("nan")) == 0
+    assert orc.sat_trunc(1e300) == 2 ** 31 - 1
+    assert orc.sat_trunc(-1e300) == -2 ** 31
+    assert orc.sat_trunc(2147483647.9) == 2147483647
+    assert orc.sat_trunc(-2147483648.9) == -2147483648
+    assert orc.sat_trunc(-0.9) == 0 and orc.sat_trunc(3.99) == 3
+    pos = np.array([[np.nan, np.inf, -np.inf, 1e300], [0, 0, 0, 0], [0, 0, 0, 0.0]])
+    out = orc.p2mesh_bin([-1] * 3, [0.5] * 3, [4] * 3, pos, np.ones(4))
+    ix = out["cell_id"] % 4
+    np.testing.assert_array_equal(ix, [0, 3, 0, 3])
+
+
+def test_golden_cells_of_reference_stars():
+    g = np.load(os.path.join(GOLDEN, "p2mesh_golden.npz"))
+    pos = g["pos"]
+    m = np.ones(pos.shape[1])
+    o200 = orc.p2mesh_bin([-20.0] * 3, [0.2] * 3, [200] * 3, pos, m)
+    np.testing.assert_array_equal(o200["cell_id"], g["cell_200"])
+    o64 = orc.p2mesh_bin([-2.0] * 3, [4.0 / 64] * 3, [64] * 3, pos, m)
+    np.testing.assert_array_equal(o64["cell_id"], g["cell_64"])
+
+
+def test_invalid_grids_rejected():
+    pos, m = _random(10, 0)
+    with pytest.raises(ValueError):
+        orc.p2mesh_bin([0] * 3, [1] * 3, [0, 4, 4], pos, m)
+    with pytest.raises(ValueError):
+        orc.p2mesh_bin([0] * 3, [1] * 3, [1, 4, 4], pos, m)  # index escapes the grid
+
+
+def test_empty_input():
+    out = orc.p2mesh_bin([-1] * 3, [0.5] * 3, [4] * 3, np.zeros((3, 0)), np.zeros(0))
+    assert out["count"].sum() == 0 and np.all(out["offset"] == 0)
