@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--quick", action="store_true", help="headline only (no extras)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=2)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 512 MiB device copy first (counter calibration under rocprof)")
     return ap.parse_args()
 
 
@@ -267,11 +269,23 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.calib_copy:
+        # a plain copy of known size: 512 MiB read + 512 MiB written
+        a = torch.ones(128 * 1024 * 1024, device="cuda:0")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log("calibration copy: 536870912 bytes read + written")
     log("generating galaxy")
     ps, vs, pd, vd = ics.galaxy(args.n_stars, args.n_dm, seed=12345)
     pos, vel, mass, h = ics.system_arrays(ps, vs, pd, vd, 1.0, 6.699356, 0.004, 0.004)
     n = pos.shape[1]
     ctx = khrr.Context(0)
+    # launch shape from the project's autotuner (cached in the context for
+    # this size bucket; every later khrr_accel of this size uses it)
+    tune, tune_ms = ctx.tune_gravity(n)
+    log(f"autotuned launch shape {tune.as_dict()} ({tune_ms:.2f} ms per trial)")
     sim = Sim(ctx, pos, vel, mass, h, 0.002)
     sim.init_forces()
     torch.cuda.synchronize()
@@ -318,7 +332,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "disk+halo galaxy, 262144 stars + 524288 halo, eps 0.004, "
                                "dt 0.002, FP32 direct sum, FP64 predictor-corrector state",
-                   "n_bodies": n, "dt": 0.002, "seed": 12345},
+                   "n_bodies": n, "dt": 0.002, "seed": 12345,
+                   "launch_shape": tune.as_dict()},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic, "kernel": "grav_kernel",
@@ -338,6 +353,7 @@ def main():
             result["cpu_baseline"] = cpu
             result["accuracy"] = acc
     if not args.quick:
+        ctx.set_timing(True)
         log("binning")
         cell_mass, binning = bench_binning(ctx, sim)
         result["binning"] = binning

@@ -18,13 +18,12 @@
 
 namespace {
 
-using khrr::fold;
-using khrr::run_pairs;
+using khrr::Accum;
 
 template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
 __global__ __launch_bounds__(256) void batched_kernel(
     int n_items, const int2* __restrict__ items, const int* __restrict__ sys_off,
-    const int* __restrict__ sys_uniform, int n_total, const float* __restrict__ pos,
+    const int* __restrict__ sys_flags, int n_total, const float* __restrict__ pos,
     const float* __restrict__ hh_all, const float* __restrict__ m, int j_tile,
     int* __restrict__ counter, float* __restrict__ acc, float* __restrict__ phi,
     int accumulate) {
@@ -40,7 +39,7 @@ __global__ __launch_bounds__(256) void batched_kernel(
     const int2 it = items[item];
     const int base = sys_off[it.x];
     const int n = sys_off[it.x + 1] - base;
-    const int uni = sys_uniform[it.x];
+    const int flags = sys_flags[it.x];
 
     float xi[IPL], yi[IPL], zi[IPL], hi[IPL];
     int idx[IPL];
@@ -54,83 +53,46 @@ __global__ __launch_bounds__(256) void batched_kernel(
       zi[k] = ok ? za[idx[k]] : 0.f;
       hi[k] = ok ? hh_all[idx[k]] : 1.f;
     }
-    float ax[IPL], ay[IPL], az[IPL], ap[IPL], lx[IPL], ly[IPL], lz[IPL], lp[IPL];
-#pragma unroll
-    for (int k = 0; k < IPL; ++k) {
-      ax[k] = ay[k] = az[k] = ap[k] = 0.f;
-      lx[k] = ly[k] = lz[k] = lp[k] = 0.f;
-    }
-    float hu[IPL];
-    if (uni) {
-      const float h0 = hh_all[base];
-#pragma unroll
-      for (int k = 0; k < IPL; ++k) hu[k] = hi[k] + h0;
-    }
+    Accum<IPL> a;
+    a.zero();
     const int j1 = base + n;
     for (int jt = base; jt < j1; jt += j_tile) {
       const int je = min(j1, jt + j_tile);
-      float tx[IPL], ty[IPL], tz[IPL], tp[IPL];
-#pragma unroll
-      for (int k = 0; k < IPL; ++k) tx[k] = ty[k] = tz[k] = tp[k] = 0.f;
-      if (uni)
-        run_pairs<IPL, DO_ACC, DO_PHI, true>(jt, je, xa, ya, za, m, hh_all, xi, yi, zi, hu, tx,
-                                             ty, tz, tp);
-      else
-        run_pairs<IPL, DO_ACC, DO_PHI, false>(jt, je, xa, ya, za, m, hh_all, xi, yi, zi, hi, tx,
-                                              ty, tz, tp);
-#pragma unroll
-      for (int k = 0; k < IPL; ++k) {
-        if (DO_ACC) {
-          fold(ax[k], lx[k], tx[k], COMP);
-          fold(ay[k], ly[k], ty[k], COMP);
-          fold(az[k], lz[k], tz[k], COMP);
-        }
-        if (DO_PHI) fold(ap[k], lp[k], tp[k], COMP);
-      }
+      khrr::tile_step<IPL, DO_ACC, DO_PHI, COMP>(jt, je, flags, xa, ya, za, m, hh_all, xi, yi, zi,
+                                                 hi, a);
     }
 #pragma unroll
-    for (int k = 0; k < IPL; ++k) {
-      const int i = idx[k];
-      if (DO_ACC && i >= 0) {
-        const float fx = COMP ? ax[k] + lx[k] : ax[k];
-        const float fy = COMP ? ay[k] + ly[k] : ay[k];
-        const float fz = COMP ? az[k] + lz[k] : az[k];
-        float* a0 = acc + i;
-        float* a1 = acc + (size_t)n_total + i;
-        float* a2 = acc + 2 * (size_t)n_total + i;
-        if (accumulate) {
-          *a0 = *a0 + fx;
-          *a1 = *a1 + fy;
-          *a2 = *a2 + fz;
-        } else {
-          *a0 = fx;
-          *a1 = fy;
-          *a2 = fz;
-        }
-      }
-      if (DO_PHI && i >= 0) {
-        const float fp = COMP ? ap[k] + lp[k] : ap[k];
-        phi[i] = accumulate ? phi[i] - fp : -fp;
-      }
-    }
+    for (int k = 0; k < IPL; ++k)
+      if (idx[k] >= 0)
+        khrr::store_final<IPL, DO_ACC, DO_PHI, COMP>(a, k, idx[k], (size_t)n_total, acc, phi,
+                                                     accumulate);
   }
 }
 
-// sys_uniform[s] = 1 when all h of system s are bitwise equal.
+// sys_flags[s]: kUniformH / kUniformM when all h / all m of system s are
+// bitwise equal.  A system's flags depend on its own bodies only, so its
+// result does not depend on the rest of the batch.
 __global__ __launch_bounds__(256) void sys_uniform_kernel(int n_sys, const int* __restrict__ off,
                                                           const float* __restrict__ h,
+                                                          const float* __restrict__ m,
                                                           int* __restrict__ flags) {
   const int lane = threadIdx.x & (khrr::kWave - 1);
   const int s = blockIdx.x * (blockDim.x / khrr::kWave) + threadIdx.x / khrr::kWave;
   if (s >= n_sys) return;
   const int b = off[s], e = off[s + 1];
-  bool same = true;
+  bool same_h = true, same_m = true;
   if (e > b) {
-    const unsigned ref = __float_as_uint(h[b]);
-    for (int j = b + lane; j < e; j += khrr::kWave) same &= __float_as_uint(h[j]) == ref;
+    const unsigned href = __float_as_uint(h[b]);
+    const unsigned mref = __float_as_uint(m[b]);
+    for (int j = b + lane; j < e; j += khrr::kWave) {
+      same_h &= __float_as_uint(h[j]) == href;
+      same_m &= __float_as_uint(m[j]) == mref;
+    }
   }
-  const unsigned long long bad = __ballot(!same);
-  if (lane == 0) flags[s] = bad == 0ull ? 1 : 0;
+  const unsigned long long bad_h = __ballot(!same_h);
+  const unsigned long long bad_m = __ballot(!same_m);
+  if (lane == 0)
+    flags[s] = (bad_h == 0ull ? khrr::kUniformH : 0) | (bad_m == 0ull ? khrr::kUniformM : 0);
 }
 
 __global__ void set_word(int* __restrict__ p, int v) { *p = v; }
@@ -166,7 +128,8 @@ BatchLauncher pick_batched(int ipl, bool a, bool p, bool c) {
 
 // Builds (or reuses) the device work list for this composition.
 int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_off, const int* dev_off,
-                  const float* h, int ipl, int2** items, int** uni, int* n_items) {
+                  const float* h, const float* m, int ipl, int2** items, int** uni,
+                  int* n_items) {
   const bool same = ctx->batch_ipl == ipl && (int)ctx->batch_off.size() == n_sys + 1 &&
                     std::equal(ctx->batch_off.begin(), ctx->batch_off.end(), host_off);
   if (!same) {
@@ -212,10 +175,10 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_off, const int* dev_
                                 khrr::Carver::need(ctx->batch_items > 0 ? ctx->batch_items : 1,
                                                    sizeof(int2)));
   *n_items = ctx->batch_items;
-  // Softening can change between calls with the same composition.
+  // Softening and masses can change between calls with the same composition.
   if (n_sys > 0)
     hipLaunchKernelGGL(sys_uniform_kernel, dim3(khrr::ceil_div(n_sys, 4)), dim3(256), 0,
-                       ctx->stream, n_sys, dev_off, h, *uni);
+                       ctx->stream, n_sys, dev_off, h, m, *uni);
   return khrr::check_launch(ctx, "sys_uniform_kernel");
 }
 
@@ -227,7 +190,7 @@ int batched_launch(khrr_ctx* ctx, int n_sys, const int* sys_off, const int* host
   int2* items = nullptr;
   int* uni = nullptr;
   int n_items = 0;
-  KHRR_TRY(prepare_items(ctx, n_sys, host_sys_off, sys_off, h, t.i_per_lane, &items, &uni,
+  KHRR_TRY(prepare_items(ctx, n_sys, host_sys_off, sys_off, h, m, t.i_per_lane, &items, &uni,
                          &n_items));
   BatchLauncher fn = pick_batched(t.i_per_lane, acc != nullptr, phi != nullptr,
                                   (flags & KHRR_COMPENSATED) != 0);

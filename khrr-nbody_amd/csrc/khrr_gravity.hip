@@ -28,8 +28,8 @@
 
 namespace {
 
+using khrr::Accum;
 using khrr::fold;
-using khrr::run_pairs;
 
 // One wave = (i-group, j-range).  Writes either the final outputs (part ==
 // nullptr) or its partial sums part[(s*NC + c)*n_i + i], NC = 4 (8 when
@@ -38,7 +38,7 @@ template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
 __global__ __launch_bounds__(256) void grav_kernel(
     int n_i, const float* __restrict__ pos_i, const float* __restrict__ h_i, int n_j,
     const float* __restrict__ pos_j, const float* __restrict__ m_j,
-    const float* __restrict__ h_j, const int* __restrict__ tile_uniform, int groups_i,
+    const float* __restrict__ h_j, const int* __restrict__ tile_flags, int groups_i,
     int n_split, int j_chunk, int j_tile, float* __restrict__ acc, float* __restrict__ phi,
     float* __restrict__ part, int accumulate) {
   const int lane = threadIdx.x & (khrr::kWave - 1);
@@ -67,84 +67,36 @@ __global__ __launch_bounds__(256) void grav_kernel(
     hi[k] = ok ? h_i[idx[k]] : 1.f;
   }
 
-  float ax[IPL], ay[IPL], az[IPL], ap[IPL];
-  float lx[IPL], ly[IPL], lz[IPL], lp[IPL];
-#pragma unroll
-  for (int k = 0; k < IPL; ++k) {
-    ax[k] = ay[k] = az[k] = ap[k] = 0.f;
-    lx[k] = ly[k] = lz[k] = lp[k] = 0.f;
-  }
-
+  Accum<IPL> a;
+  a.zero();
   for (int jt = j0; jt < j1; jt += j_tile) {
     const int je = min(j1, jt + j_tile);
-    float tx[IPL], ty[IPL], tz[IPL], tp[IPL];
-#pragma unroll
-    for (int k = 0; k < IPL; ++k) tx[k] = ty[k] = tz[k] = tp[k] = 0.f;
-    if (tile_uniform[jt / j_tile]) {
-      const float h0 = h_j[jt];
-      float hh[IPL];
-#pragma unroll
-      for (int k = 0; k < IPL; ++k) hh[k] = hi[k] + h0;
-      run_pairs<IPL, DO_ACC, DO_PHI, true>(jt, je, xj, yj, zj, m_j, h_j, xi, yi, zi, hh, tx, ty,
-                                           tz, tp);
-    } else {
-      run_pairs<IPL, DO_ACC, DO_PHI, false>(jt, je, xj, yj, zj, m_j, h_j, xi, yi, zi, hi, tx, ty,
-                                            tz, tp);
-    }
-#pragma unroll
-    for (int k = 0; k < IPL; ++k) {
-      if (DO_ACC) {
-        fold(ax[k], lx[k], tx[k], COMP);
-        fold(ay[k], ly[k], ty[k], COMP);
-        fold(az[k], lz[k], tz[k], COMP);
-      }
-      if (DO_PHI) fold(ap[k], lp[k], tp[k], COMP);
-    }
+    khrr::tile_step<IPL, DO_ACC, DO_PHI, COMP>(jt, je, tile_flags[jt / j_tile], xj, yj, zj, m_j,
+                                               h_j, xi, yi, zi, hi, a);
   }
 
 #pragma unroll
   for (int k = 0; k < IPL; ++k) {
     const int i = idx[k];
-    if (i >= n_i) continue;
-    if (part) {
+    if (i < n_i && part) {
       constexpr int NC = COMP ? 8 : 4;
       float* p = part + (size_t)s * NC * n_i;
       if (DO_ACC) {
-        p[i] = ax[k];
-        p[(size_t)n_i + i] = ay[k];
-        p[2 * (size_t)n_i + i] = az[k];
+        p[i] = a.ax[k];
+        p[(size_t)n_i + i] = a.ay[k];
+        p[2 * (size_t)n_i + i] = a.az[k];
       }
-      if (DO_PHI) p[3 * (size_t)n_i + i] = ap[k];
+      if (DO_PHI) p[3 * (size_t)n_i + i] = a.ap[k];
       if (COMP) {
         if (DO_ACC) {
-          p[4 * (size_t)n_i + i] = lx[k];
-          p[5 * (size_t)n_i + i] = ly[k];
-          p[6 * (size_t)n_i + i] = lz[k];
+          p[4 * (size_t)n_i + i] = a.lx[k];
+          p[5 * (size_t)n_i + i] = a.ly[k];
+          p[6 * (size_t)n_i + i] = a.lz[k];
         }
-        if (DO_PHI) p[7 * (size_t)n_i + i] = lp[k];
+        if (DO_PHI) p[7 * (size_t)n_i + i] = a.lp[k];
       }
-    } else {
-      if (DO_ACC) {
-        const float fx = COMP ? ax[k] + lx[k] : ax[k];
-        const float fy = COMP ? ay[k] + ly[k] : ay[k];
-        const float fz = COMP ? az[k] + lz[k] : az[k];
-        float* a0 = acc + i;
-        float* a1 = acc + (size_t)n_i + i;
-        float* a2 = acc + 2 * (size_t)n_i + i;
-        if (accumulate) {
-          *a0 = *a0 + fx;
-          *a1 = *a1 + fy;
-          *a2 = *a2 + fz;
-        } else {
-          *a0 = fx;
-          *a1 = fy;
-          *a2 = fz;
-        }
-      }
-      if (DO_PHI) {
-        const float fp = COMP ? ap[k] + lp[k] : ap[k];
-        phi[i] = accumulate ? phi[i] - fp : -fp;
-      }
+    } else if (i < n_i) {
+      khrr::store_final<IPL, DO_ACC, DO_PHI, COMP>(a, k, i, (size_t)n_i, acc, phi, accumulate);
     }
   }
 }
@@ -179,9 +131,10 @@ __global__ __launch_bounds__(256) void grav_reduce(int n_i, int n_split, int do_
   }
 }
 
-// tile_uniform[t] = 1 when every h_j of j-tile t equals the tile's first
-// (bitwise), so h_i + h_j can be hoisted out of the pair loop for it.
+// tile_flags[t]: kUniformH when every h_j of j-tile t equals the tile's
+// first (bitwise), kUniformM likewise for m_j.
 __global__ __launch_bounds__(256) void tile_uniform_kernel(int n_j, const float* __restrict__ h_j,
+                                                           const float* __restrict__ m_j,
                                                            int j_tile, int n_tiles,
                                                            int* __restrict__ flags) {
   const int lane = threadIdx.x & (khrr::kWave - 1);
@@ -189,11 +142,17 @@ __global__ __launch_bounds__(256) void tile_uniform_kernel(int n_j, const float*
   if (t >= n_tiles) return;
   const int j0 = t * j_tile;
   const int j1 = min(n_j, j0 + j_tile);
-  const unsigned ref = __float_as_uint(h_j[j0]);
-  bool same = true;
-  for (int j = j0 + lane; j < j1; j += khrr::kWave) same &= (__float_as_uint(h_j[j]) == ref);
-  const unsigned long long bad = __ballot(!same);
-  if (lane == 0) flags[t] = bad == 0ull ? 1 : 0;
+  const unsigned href = __float_as_uint(h_j[j0]);
+  const unsigned mref = __float_as_uint(m_j[j0]);
+  bool same_h = true, same_m = true;
+  for (int j = j0 + lane; j < j1; j += khrr::kWave) {
+    same_h &= __float_as_uint(h_j[j]) == href;
+    same_m &= __float_as_uint(m_j[j]) == mref;
+  }
+  const unsigned long long bad_h = __ballot(!same_h);
+  const unsigned long long bad_m = __ballot(!same_m);
+  if (lane == 0)
+    flags[t] = (bad_h == 0ull ? khrr::kUniformH : 0) | (bad_m == 0ull ? khrr::kUniformM : 0);
 }
 
 __global__ void fill_kernel(float* __restrict__ p, size_t n, float v) {
@@ -330,7 +289,7 @@ int accel_launch(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, i
   {
     Timed tm(ctx, KHRR_K_GRAVITY);
     hipLaunchKernelGGL(tile_uniform_kernel, dim3(ceil_div(n_tiles, 4)), dim3(256), 0, ctx->stream,
-                       n_j, h_j, j_tile, n_tiles, tile_flags);
+                       n_j, h_j, m_j, j_tile, n_tiles, tile_flags);
     fn(dim3(ceil_div(waves, wpb)), dim3(wpb * 64), ctx->stream, n_i, pos_i, h_i, n_j, pos_j, m_j,
        h_j, tile_flags, groups_i, n_split, (int)chunk, j_tile, acc, phi, part, accumulate);
     if (n_split > 1) {
@@ -364,6 +323,7 @@ int khrr_accel(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int
   if (n_i < 0 || n_j < 0) return khrr::fail(ctx, KHRR_EINVAL, "negative size (n_i=%d n_j=%d)", n_i, n_j);
   if (flags & ~(KHRR_ACCUMULATE | KHRR_COMPENSATED))
     return khrr::fail(ctx, KHRR_EINVAL, "unknown flags 0x%x", flags);
+  if (n_i == 0) return 0;  // no i-particles: nothing to write
   if (!acc && !phi) return khrr::fail(ctx, KHRR_EINVAL, "acc and phi are both NULL");
   if (n_i > 0 && (!pos_i || !h_i)) return khrr::fail(ctx, KHRR_EINVAL, "pos_i/h_i is NULL");
   if (n_j > 0 && (!pos_j || !m_j || !h_j))

@@ -173,21 +173,38 @@ __device__ __forceinline__ int lower_bound_dev(const int* __restrict__ a, int n,
   return lo;
 }
 
+// bounds[b] = number of sorted particles with cell < b*512, for b in [0, B]:
+// all block boundaries found by independent binary searches in one launch.
+__global__ __launch_bounds__(256) void block_bounds(int n, int nb, const int* __restrict__ skeys,
+                                                    int* __restrict__ bounds) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b <= nb) bounds[b] = lower_bound_dev(skeys, n, (long long)b * kCellBlock);
+}
+
 __global__ __launch_bounds__(kCellBlock) void bin_cells(
     int n, long long G, const int* __restrict__ skeys, const int* __restrict__ svals,
-    const double* __restrict__ mass, int* __restrict__ count, int* __restrict__ offset,
-    int* __restrict__ start_ref, int* __restrict__ block_incl, int* __restrict__ id_in_cell,
-    double* __restrict__ cell_mass) {
+    const int* __restrict__ bounds, const double* __restrict__ mass, int* __restrict__ count,
+    int* __restrict__ offset, int* __restrict__ start_ref, int* __restrict__ block_incl,
+    int* __restrict__ id_in_cell, double* __restrict__ cell_mass) {
   __shared__ int cnt[kCellBlock];
   __shared__ int wsum[kCellBlock / khrr::kWave];
-  __shared__ int bounds[2];
   const int tid = threadIdx.x;
   const long long c0 = (long long)blockIdx.x * kCellBlock;
-  if (tid == 0) bounds[0] = lower_bound_dev(skeys, n, c0);
-  if (tid == khrr::kWave) bounds[1] = lower_bound_dev(skeys, n, c0 + kCellBlock);
+  const long long c = c0 + tid;
+  const int lo = bounds[blockIdx.x], hi = bounds[blockIdx.x + 1];
+  if (tid == 0 && block_incl) block_incl[blockIdx.x] = hi;
+  if (lo == hi) {
+    // empty 512-cell block (most of a fine grid): plain coalesced stores
+    if (c < G) {
+      count[c] = 0;
+      if (offset) offset[c] = lo;
+      if (start_ref) start_ref[c] = lo;
+      if (cell_mass) cell_mass[c] = 0.0;
+    }
+    return;
+  }
   cnt[tid] = 0;
   __syncthreads();
-  const int lo = bounds[0], hi = bounds[1];
   for (int p = lo + tid; p < hi; p += kCellBlock) atomicAdd(&cnt[skeys[p] - (int)c0], 1);
   __syncthreads();
   // Exclusive scan of cnt: wave-inclusive scans, then wave totals.
@@ -205,7 +222,6 @@ __global__ __launch_bounds__(kCellBlock) void bin_cells(
   int before = 0;
   for (int w = 0; w < wave; ++w) before += wsum[w];
   const int excl = lo + before + incl - mine;
-  const long long c = c0 + tid;
   if (c < G) {
     count[c] = mine;
     if (offset) offset[c] = excl;
@@ -218,7 +234,6 @@ __global__ __launch_bounds__(kCellBlock) void bin_cells(
     }
     if (cell_mass) cell_mass[c] = m;
   }
-  if (tid == 0 && block_incl) block_incl[blockIdx.x] = hi;
 }
 
 // perm[offset[cell_id[i]] + id_in_cell[i]] = i: the sorted segments back
@@ -307,8 +322,10 @@ extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, cons
   if (n > 0 && cell_mass && !mass) return fail(ctx, KHRR_EINVAL, "mass is NULL");
 
   const int ntiles = n > 0 ? (int)ceil_div(n, kSortTile) : 1;
+  const int cblocks = (int)ceil_div(G, kCellBlock);
   const size_t need = 5 * Carver::need((size_t)(n > 0 ? n : 1), sizeof(int)) +
-                      Carver::need((size_t)kRadix * ntiles, sizeof(int));
+                      Carver::need((size_t)kRadix * ntiles, sizeof(int)) +
+                      Carver::need((size_t)cblocks + 1, sizeof(int));
   void* base = nullptr;
   KHRR_TRY(scratch(ctx, need, &base));
   Carver cv(base);
@@ -318,6 +335,7 @@ extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, cons
   int* kb = cv.take<int>(n > 0 ? n : 1);
   int* vb = cv.take<int>(n > 0 ? n : 1);
   int* hist = cv.take<int>((size_t)kRadix * ntiles);
+  int* bounds = cv.take<int>((size_t)cblocks + 1);
   if (cell_id) cells = cell_id;
 
   GridArgs g{grid->min[0], grid->min[1], grid->min[2], grid->cell[0], grid->cell[1],
@@ -360,8 +378,10 @@ extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, cons
       svals = va;
     }
   }
-  const unsigned cblocks = ceil_div(G, kCellBlock);
+  hipLaunchKernelGGL(block_bounds, dim3(ceil_div(cblocks + 1, 256)), dim3(256), 0, ctx->stream, n,
+                     cblocks, skeys, bounds);
   hipLaunchKernelGGL(bin_cells, dim3(cblocks), dim3(kCellBlock), 0, ctx->stream, n, G, skeys,
-                     svals, mass, count, offset, start_id_ref, block_incl, id_in_cell, cell_mass);
+                     svals, bounds, mass, count, offset, start_id_ref, block_incl, id_in_cell,
+                     cell_mass);
   return check_launch(ctx, "p2mesh kernels");
 }

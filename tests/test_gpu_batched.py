@@ -17,8 +17,8 @@ def _population(sizes, seed0=100000):
     parts = []
     for k, n in enumerate(sizes):
         pos, vel = ics.plummer(int(n), 1.0, 0.3333, 1.0, seed=seed0 + k)
-        eps = 0.004 * (n / 16384.0) ** (-1.0 / 3.0)
-        parts.append((pos, np.full(n, 1.0 / n), np.full(n, 0.5 * eps * eps)))
+        eps = 0.004 * (max(n, 1) / 16384.0) ** (-1.0 / 3.0)
+        parts.append((pos, np.full(n, 1.0 / max(n, 1)), np.full(n, 0.5 * eps * eps)))
     off = np.concatenate([[0], np.cumsum([p[0].shape[1] for p in parts])]).astype(np.int32)
     pos = np.concatenate([p[0] for p in parts], axis=1).astype(np.float32)
     m = np.concatenate([p[1] for p in parts]).astype(np.float32)

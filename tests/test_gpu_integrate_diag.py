@@ -79,10 +79,18 @@ def test_diag_matches_oracle(ctx):
     out = ctx.diag(to_dev(pos, torch.float64), to_dev(vel, torch.float64),
                    to_dev(mass, torch.float64), to_dev(phi, torch.float32))
     ref = orc.diag(pos, vel, mass, phi.astype(np.float64))
-    mags = orc.diag(np.abs(pos), np.abs(vel), mass, np.abs(phi).astype(np.float64))
+    # scale of each sum: sum of |terms| (products taken term by term, so the
+    # cross-product components do not cancel in the scale)
+    x, y, z = np.abs(pos)
+    vx, vy, vz = np.abs(vel)
+    scale = [np.sum(mass * (vx * vx + vy * vy + vz * vz)), np.sum(mass * np.abs(phi)),
+             np.sum(mass * vx), np.sum(mass * vy), np.sum(mass * vz),
+             np.sum(mass * (vz * y + vy * z)), np.sum(mass * (vx * z + vz * x)),
+             np.sum(mass * (vy * x + vx * y)), np.sum(mass), np.sum(mass * x),
+             np.sum(mass * y), np.sum(mass * z)]
     for k, name in enumerate(khrr.DIAG_NAMES):
         if k < 12:
-            assert abs(out[name] - ref[k]) <= 1e-12 * max(abs(mags[k]), 1e-300) + 1e-300, name
+            assert abs(out[name] - ref[k]) <= 1e-13 * scale[k], name
         else:
             assert out[name] == ref[k], name
 
