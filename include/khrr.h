@@ -96,9 +96,14 @@ int khrr_ctx_reset_timing(khrr_ctx* ctx);
  * "pick automatically" (see khrr_tune_gravity). */
 typedef struct {
   int i_per_lane;   /* i-particles held per lane: 1, 2, 4 or 8          */
-  int waves_per_block; /* 1..16                                         */
+  int waves_per_block; /* 1..4                                          */
   int j_split;      /* number of independent j-ranges (partial sums)    */
   int j_tile;       /* j count per FP32 partial-sum tile (multiple of 16) */
+  int symmetric;    /* 1: when the i-set and j-set are the same buffers,
+                       evaluate each unordered pair once and apply it to
+                       both bodies (needs i_per_lane 4 or 8 and a partial
+                       buffer of 16 B per body per 4096 bodies, at most
+                       6 GiB; otherwise the one-sided kernel runs)       */
 } khrr_tune;
 
 /* Softened direct sum.  For every i in [0,n_i):

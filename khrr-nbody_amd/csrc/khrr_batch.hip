@@ -309,7 +309,7 @@ int khrr_tune_gravity(khrr_ctx* ctx, int n, khrr_tune* best, double* best_ms) {
       for (int sp : splits) {
         if (rc) break;
         if (sp > 1 && (long long)nj / sp < 1024) continue;
-        khrr_tune c{ipl, wpb, sp, 256};
+        khrr_tune c{ipl, wpb, sp, 256, 0};
         double ms = 0.0;
         rc = time_candidate(ctx, [&] {
           return khrr::accel_launch(ctx, n, pos, h, nj, pos, m, h, acc, nullptr, 0, c);
@@ -319,6 +319,22 @@ int khrr_tune_gravity(khrr_ctx* ctx, int n, khrr_tune* best, double* best_ms) {
           best_c = c;
         }
       }
+    }
+  }
+  // Pair-symmetric candidates run on the full system (i-set == j-set) and
+  // are compared per pair-interaction with the one-sided ones.
+  double best_per_pair = best_t / ((double)n * nj);
+  for (int ipl : {4, 8}) {
+    if (rc || !khrr::sym_eligible(n, ipl)) break;
+    khrr_tune c{ipl, best_c.waves_per_block, best_c.j_split, 256, 1};
+    double ms = 0.0;
+    rc = time_candidate(ctx, [&] {
+      return khrr::accel_launch(ctx, n, pos, h, n, pos, m, h, acc, nullptr, 0, c);
+    }, &ms);
+    if (!rc && ms / ((double)n * n) < best_per_pair) {
+      best_per_pair = ms / ((double)n * n);
+      best_t = ms;
+      best_c = c;
     }
   }
   ctx->timing = timing;

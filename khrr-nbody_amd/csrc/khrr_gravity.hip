@@ -213,18 +213,22 @@ int validate_tune(khrr_ctx* ctx, const khrr_tune& t) {
   if (t.j_tile < 16 || t.j_tile % 16 != 0 || t.j_tile > (1 << 20))
     return fail(ctx, KHRR_EINVAL, "tune.j_tile must be a positive multiple of 16 (got %d)",
                 t.j_tile);
+  if (t.symmetric != 0 && t.symmetric != 1)
+    return fail(ctx, KHRR_EINVAL, "tune.symmetric must be 0 or 1 (got %d)", t.symmetric);
   return 0;
 }
 
-// Static launch shape: 4 i per lane, 4 waves per block, 256-j tiles and a
-// j-split that gives at least ~2 waves per SIMD (4 SIMDs per CU).
+// Static launch shape: the pair-symmetric kernel (4 i per lane) where it
+// applies; otherwise 2 i per lane, 4 waves per block, 256-j tiles and a
+// j-split that gives ~8 waves per SIMD (4 SIMDs per CU).
 khrr_tune default_tune(const khrr_ctx* ctx, int n_i, int n_j) {
   khrr_tune t;
-  t.i_per_lane = 4;
+  t.i_per_lane = 2;
   t.waves_per_block = 4;
   t.j_tile = 256;
+  t.symmetric = 0;
   const long long groups = (n_i + 64ll * t.i_per_lane - 1) / (64ll * t.i_per_lane);
-  const long long target = 8ll * ctx->cu_count;
+  const long long target = 32ll * ctx->cu_count;
   long long split = groups > 0 ? (target + groups - 1) / groups : 1;
   const long long max_split = n_j / 1024 > 1 ? n_j / 1024 : 1;
   if (split > max_split) split = max_split;
@@ -265,6 +269,9 @@ int accel_launch(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, i
     }
     return 0;
   }
+  if (tune.symmetric && n_i == n_j && pos_i == pos_j && h_i == h_j &&
+      sym_eligible(n_i, tune.i_per_lane))
+    return sym_launch(ctx, n_i, pos_i, h_i, m_j, acc, phi, flags, tune.i_per_lane);
   const int ipl = tune.i_per_lane;
   const int groups_i = (int)ceil_div(n_i, 64ll * ipl);
   const int j_tile = tune.j_tile;
@@ -335,6 +342,11 @@ int khrr_accel(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int
     t = *c;
   } else {
     t = khrr::default_tune(ctx, n_i, n_j);
+    const bool same_set = n_i == n_j && pos_i == pos_j && h_i == h_j;
+    if (same_set && khrr::sym_eligible(n_i, 4)) {
+      t.i_per_lane = 4;
+      t.symmetric = 1;
+    }
   }
   KHRR_TRY(khrr::validate_tune(ctx, t));
   return khrr::accel_launch(ctx, n_i, pos_i, h_i, n_j, pos_j, m_j, h_j, acc, phi, flags, t);

@@ -115,6 +115,10 @@ void store_tune(khrr_ctx* ctx, int n, const khrr_tune& t);
 int accel_launch(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int n_j,
                  const float* pos_j, const float* m_j, const float* h_j, float* acc, float* phi,
                  unsigned flags, const khrr_tune& tune);
+// Pair-symmetric form (khrr_sym.hip) for one system (i-set == j-set).
+bool sym_eligible(int n, int ipl);
+int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const float* m, float* acc,
+               float* phi, unsigned flags, int ipl);
 
 }  // namespace khrr
 

@@ -1,0 +1,319 @@
+// khrr_sym.hip -- direct-sum gravity with each unordered pair evaluated once.
+//
+// Same results as ACCEL_kernel / PHI_kernel (nbody-kernel.cuh:43-142) when
+// the i-set and the j-set are one system: the pair force is antisymmetric
+// under the reference's symmetric softening 0.5*(eps2_i + eps2_j), so one
+// rsq serves both a_i and a_j (Newton's third law).  That halves the
+// transcendental, subtraction and r^2 work per directed pair.
+//
+// Work decomposition (gfx950):
+//  * bodies are cut into superblocks of kSB = 4096; a workgroup takes one
+//    item (A, B) of the upper triangle, A <= B;
+//  * off-diagonal items: each wave owns 64*IPL i-bodies of A; the j-bodies of
+//    B stream through in chunks of 64, one per lane, and rotate one lane per
+//    step (DPP wave_ror:1) together with their accumulators, so after 64
+//    steps every lane-i has met every chunk-j once and each j-accumulator is
+//    back in its lane.  The W waves' j-accumulators are then added in LDS in
+//    wave order;
+//  * diagonal items (A, A) run the one-sided scalar-load loop;
+//  * every (superblock X, body i) partial slot part[X][c][i] is written by
+//    exactly one item, and a second kernel adds the slots in X order: the
+//    result is deterministic and needs no atomics.
+// Per unique pair with uniform mass and softening: 14 VALU ops + one
+// v_rsq_f32 (+6 lane rotations per IPL pairs), i.e. about half of the
+// one-sided cost per directed pair.
+#include "khrr_internal.h"
+#include "khrr_pair.h"
+
+namespace {
+
+using khrr::Accum;
+using khrr::fold;
+using khrr::kUniformH;
+using khrr::kUniformM;
+
+constexpr int kSB = 4096;  // bodies per superblock
+
+__device__ __forceinline__ float ror1(float v) {
+  // DPP wave_ror:1 (dpp_ctrl 0x13C): every lane receives its neighbour's
+  // value (all rows and banks enabled, so `old` is never selected; passing
+  // the source as `old` lets the move happen in place).
+  const int b = __builtin_bit_cast(int, v);
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x13C, 0xF, 0xF, false));
+}
+
+template <bool DO_ACC, bool DO_PHI, bool UNIH, bool UNIM>
+__device__ __forceinline__ void pair_sym(float xj, float yj, float zj, float mj, float hj, float xi,
+                                         float yi, float zi, float hi, float mi, float& tx,
+                                         float& ty, float& tz, float& tp, float& bx, float& by,
+                                         float& bz, float& bp) {
+  const float dx = xj - xi;
+  const float dy = yj - yi;
+  const float dz = zj - zi;
+  const float h = UNIH ? hi : hi + hj;  // UNIH: hi already holds h_i + h_j
+  const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, h)));
+  const float inv = __builtin_amdgcn_rsqf(r2);
+  const float inv3 = inv * (inv * inv);
+  if (DO_ACC) {
+    const float si = UNIM ? inv3 : mj * inv3;
+    const float sj = UNIM ? inv3 : mi * inv3;
+    tx = __builtin_fmaf(dx, si, tx);
+    ty = __builtin_fmaf(dy, si, ty);
+    tz = __builtin_fmaf(dz, si, tz);
+    bx = __builtin_fmaf(dx, sj, bx);
+    by = __builtin_fmaf(dy, sj, by);
+    bz = __builtin_fmaf(dz, sj, bz);
+  }
+  if (DO_PHI) {
+    tp += UNIM ? inv : mj * inv;
+    bp += UNIM ? inv : mi * inv;
+  }
+}
+
+// 64 rotation steps over one chunk of j-bodies held one per lane.
+template <int IPL, bool DO_ACC, bool DO_PHI, bool UNIH, bool UNIM>
+__device__ __forceinline__ void chunk64(float& xj, float& yj, float& zj, float& mj, float& hj,
+                                        const float (&xi)[IPL], const float (&yi)[IPL],
+                                        const float (&zi)[IPL], const float (&hi)[IPL],
+                                        const float (&mi)[IPL], float (&tx)[IPL],
+                                        float (&ty)[IPL], float (&tz)[IPL], float (&tp)[IPL],
+                                        float& bx, float& by, float& bz, float& bp) {
+#pragma unroll 4
+  for (int s = 0; s < khrr::kWave; ++s) {
+#pragma unroll
+    for (int k = 0; k < IPL; ++k)
+      pair_sym<DO_ACC, DO_PHI, UNIH, UNIM>(xj, yj, zj, mj, hj, xi[k], yi[k], zi[k], hi[k], mi[k],
+                                           tx[k], ty[k], tz[k], tp[k], bx, by, bz, bp);
+    xj = ror1(xj);
+    yj = ror1(yj);
+    zj = ror1(zj);
+    if (!UNIM) mj = ror1(mj);
+    if (!UNIH) hj = ror1(hj);
+    if (DO_ACC) {
+      bx = ror1(bx);
+      by = ror1(by);
+      bz = ror1(bz);
+    }
+    if (DO_PHI) bp = ror1(bp);
+  }
+}
+
+// part layout: part[(X*4 + c)*n + i], c = ax, ay, az (acceleration from the
+// bodies of superblock X) and 3 = sum_j m_j / r (positive).
+template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
+__global__ __launch_bounds__(kSB / IPL) void sym_kernel(int n, int n_sb,
+                                                        const float* __restrict__ pos,
+                                                        const float* __restrict__ m,
+                                                        const float* __restrict__ h,
+                                                        const int* __restrict__ sb_flags,
+                                                        float* __restrict__ part) {
+  constexpr int W = kSB / (khrr::kWave * IPL);
+  __shared__ float red[W][4][khrr::kWave];
+  // item -> (A, B), row-major over the upper triangle
+  int t = blockIdx.x, A = 0, row = n_sb;
+  while (t >= row) {
+    t -= row;
+    ++A;
+    --row;
+  }
+  const int B = A + t;
+  const int lane = threadIdx.x & (khrr::kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / khrr::kWave);
+  const size_t N = (size_t)n;
+  const float* xs = pos;
+  const float* ys = pos + N;
+  const float* zs = pos + 2 * N;
+  const int flags = sb_flags[A] & sb_flags[B];
+
+  float xi[IPL], yi[IPL], zi[IPL], hi[IPL], mi[IPL];
+  int idx[IPL];
+#pragma unroll
+  for (int k = 0; k < IPL; ++k) {
+    idx[k] = A * kSB + wave * (khrr::kWave * IPL) + k * khrr::kWave + lane;
+    const bool ok = idx[k] < n;
+    // an absent i sits at 1e30: r^2 overflows to inf, rsq gives 0 and every
+    // term it adds to a j-body is exactly 0
+    xi[k] = ok ? xs[idx[k]] : 1e30f;
+    yi[k] = ok ? ys[idx[k]] : 1e30f;
+    zi[k] = ok ? zs[idx[k]] : 1e30f;
+    hi[k] = ok ? h[idx[k]] : 1.f;
+    mi[k] = ok ? m[idx[k]] : 0.f;
+  }
+  Accum<IPL> acc;
+  acc.zero();
+
+  if (A == B) {
+    // diagonal: one-sided over the superblock's own bodies (j = i included)
+    const int j0 = A * kSB, j1 = min(n, j0 + kSB);
+    for (int jt = j0; jt < j1; jt += 256)
+      khrr::tile_step<IPL, DO_ACC, DO_PHI, COMP>(jt, min(j1, jt + 256), sb_flags[A], xs, ys, zs,
+                                                 m, h, xi, yi, zi, hi, acc);
+  } else {
+    const bool uh = (flags & kUniformH) != 0;
+    const bool um = (flags & kUniformM) != 0;
+    const int j0 = B * kSB, j1 = min(n, j0 + kSB);
+    const float hsum = h[A * kSB] + h[j0];
+    const float mA = m[A * kSB], mB = m[j0];
+    float hh[IPL];  // h_i + h_j of the uniform path
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) hh[k] = hsum;
+    for (int c0 = j0; c0 < j1; c0 += khrr::kWave) {
+      const int j = c0 + lane;
+      const bool vj = j < j1;
+      float xj = vj ? xs[j] : 1e30f, yj = vj ? ys[j] : 1e30f, zj = vj ? zs[j] : 1e30f;
+      float mj = vj ? m[j] : 0.f, hj = vj ? h[j] : 1.f;
+      float tx[IPL], ty[IPL], tz[IPL], tp[IPL];
+#pragma unroll
+      for (int k = 0; k < IPL; ++k) tx[k] = ty[k] = tz[k] = tp[k] = 0.f;
+      float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
+      if (uh && um)
+        chunk64<IPL, DO_ACC, DO_PHI, true, true>(xj, yj, zj, mj, hj, xi, yi, zi, hh, mi, tx, ty,
+                                                 tz, tp, bx, by, bz, bp);
+      else
+        chunk64<IPL, DO_ACC, DO_PHI, false, false>(xj, yj, zj, mj, hj, xi, yi, zi, hi, mi, tx, ty,
+                                                   tz, tp, bx, by, bz, bp);
+      const float si = um ? mB : 1.f;
+#pragma unroll
+      for (int k = 0; k < IPL; ++k) {
+        if (DO_ACC) {
+          fold(acc.ax[k], acc.lx[k], tx[k] * si, COMP);
+          fold(acc.ay[k], acc.ly[k], ty[k] * si, COMP);
+          fold(acc.az[k], acc.lz[k], tz[k] * si, COMP);
+        }
+        if (DO_PHI) fold(acc.ap[k], acc.lp[k], tp[k] * si, COMP);
+      }
+      // j-side: add the W waves' accumulators in wave order
+      red[wave][0][lane] = bx;
+      red[wave][1][lane] = by;
+      red[wave][2][lane] = bz;
+      red[wave][3][lane] = bp;
+      __syncthreads();
+      if (threadIdx.x < 4 * khrr::kWave) {
+        const int c = threadIdx.x / khrr::kWave;
+        const int l = threadIdx.x % khrr::kWave;
+        const bool want = c < 3 ? DO_ACC : DO_PHI;
+        if (want && c0 + l < j1) {
+          float s = 0.f;
+          for (int w = 0; w < W; ++w) s += red[w][c][l];
+          const float sj = um ? mA : 1.f;
+          // a_j gets -(sum m_i d_ij / r^3): d_ij points from i to j
+          part[((size_t)A * 4 + c) * N + c0 + l] = c < 3 ? -(s * sj) : s * sj;
+        }
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < IPL; ++k) {
+    if (idx[k] >= n) continue;
+    const size_t base = (size_t)B * 4 * N + idx[k];
+    if (DO_ACC) {
+      part[base] = COMP ? acc.ax[k] + acc.lx[k] : acc.ax[k];
+      part[base + N] = COMP ? acc.ay[k] + acc.ly[k] : acc.ay[k];
+      part[base + 2 * N] = COMP ? acc.az[k] + acc.lz[k] : acc.az[k];
+    }
+    if (DO_PHI) part[base + 3 * N] = COMP ? acc.ap[k] + acc.lp[k] : acc.ap[k];
+  }
+}
+
+// acc[i] (+)= sum_X part[X][0..2][i];  phi[i] (-)= sum_X part[X][3][i]
+template <bool COMP>
+__global__ __launch_bounds__(256) void sym_reduce(int n, int n_sb, int do_acc, int do_phi,
+                                                  const float* __restrict__ part,
+                                                  float* __restrict__ acc,
+                                                  float* __restrict__ phi, int accumulate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t N = (size_t)n;
+  for (int c = 0; c < 4; ++c) {
+    if ((c < 3 && !do_acc) || (c == 3 && !do_phi)) continue;
+    float hi = 0.f, lo = 0.f;
+    for (int X = 0; X < n_sb; ++X) fold(hi, lo, part[((size_t)X * 4 + c) * N + i], COMP);
+    const float v = COMP ? hi + lo : hi;
+    if (c < 3) {
+      float* a = acc + c * N + i;
+      *a = accumulate ? *a + v : v;
+    } else {
+      phi[i] = accumulate ? phi[i] - v : -v;
+    }
+  }
+}
+
+// sb_flags[X]: kUniformH / kUniformM over superblock X.
+__global__ __launch_bounds__(256) void sb_flags_kernel(int n, int n_sb, const float* __restrict__ h,
+                                                       const float* __restrict__ m,
+                                                       int* __restrict__ flags) {
+  const int lane = threadIdx.x & (khrr::kWave - 1);
+  const int X = blockIdx.x * (blockDim.x / khrr::kWave) + threadIdx.x / khrr::kWave;
+  if (X >= n_sb) return;
+  const int b = X * kSB, e = min(n, b + kSB);
+  const unsigned href = __float_as_uint(h[b]), mref = __float_as_uint(m[b]);
+  bool sh = true, sm = true;
+  for (int j = b + lane; j < e; j += khrr::kWave) {
+    sh &= __float_as_uint(h[j]) == href;
+    sm &= __float_as_uint(m[j]) == mref;
+  }
+  const unsigned long long bh = __ballot(!sh), bm = __ballot(!sm);
+  if (lane == 0) flags[X] = (bh == 0ull ? kUniformH : 0) | (bm == 0ull ? kUniformM : 0);
+}
+
+template <int IPL, bool A, bool P, bool C>
+void launch_sym(hipStream_t st, int n, int n_sb, const float* pos, const float* m, const float* h,
+                const int* flags, float* part) {
+  const unsigned items = (unsigned)((long long)n_sb * (n_sb + 1) / 2);
+  hipLaunchKernelGGL((sym_kernel<IPL, A, P, C>), dim3(items), dim3(kSB / IPL), 0, st, n, n_sb, pos,
+                     m, h, flags, part);
+}
+
+using SymLauncher = void (*)(hipStream_t, int, int, const float*, const float*, const float*,
+                             const int*, float*);
+
+template <int IPL>
+SymLauncher pick_sym_mode(bool a, bool p, bool c) {
+  if (a && p) return c ? launch_sym<IPL, true, true, true> : launch_sym<IPL, true, true, false>;
+  if (a) return c ? launch_sym<IPL, true, false, true> : launch_sym<IPL, true, false, false>;
+  return c ? launch_sym<IPL, false, true, true> : launch_sym<IPL, false, true, false>;
+}
+
+}  // namespace
+
+namespace khrr {
+
+constexpr size_t kSymPartBudget = size_t(6) << 30;  // partial-slot buffer cap (6 GiB)
+
+bool sym_eligible(int n, int ipl) {
+  if (ipl != 4 && ipl != 8) return false;
+  const long long n_sb = (n + kSB - 1) / kSB;
+  if (n_sb < 2) return false;
+  return (size_t)n_sb * 4 * (size_t)n * sizeof(float) <= kSymPartBudget;
+}
+
+int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const float* m, float* acc,
+               float* phi, unsigned flags, int ipl) {
+  const bool do_acc = acc != nullptr, do_phi = phi != nullptr;
+  const bool comp = (flags & KHRR_COMPENSATED) != 0;
+  const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
+  const int n_sb = (n + kSB - 1) / kSB;
+  const size_t part_elems = (size_t)n_sb * 4 * n;
+  void* base = nullptr;
+  KHRR_TRY(scratch(ctx, Carver::need(n_sb, sizeof(int)) + Carver::need(part_elems, sizeof(float)),
+                   &base));
+  Carver cv(base);
+  int* sbf = cv.take<int>(n_sb);
+  float* part = cv.take<float>(part_elems);
+  SymLauncher fn = ipl == 8 ? pick_sym_mode<8>(do_acc, do_phi, comp)
+                            : pick_sym_mode<4>(do_acc, do_phi, comp);
+  Timed tm(ctx, KHRR_K_GRAVITY);
+  hipLaunchKernelGGL(sb_flags_kernel, dim3(ceil_div(n_sb, 4)), dim3(256), 0, ctx->stream, n, n_sb, h,
+                     m, sbf);
+  fn(ctx->stream, n, n_sb, pos, m, h, sbf, part);
+  if (comp)
+    hipLaunchKernelGGL(sym_reduce<true>, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, n,
+                       n_sb, (int)do_acc, (int)do_phi, part, acc, phi, accumulate);
+  else
+    hipLaunchKernelGGL(sym_reduce<false>, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, n,
+                       n_sb, (int)do_acc, (int)do_phi, part, acc, phi, accumulate);
+  return check_launch(ctx, "sym_kernel");
+}
+
+}  // namespace khrr

@@ -48,7 +48,7 @@ class KhrrError(RuntimeError):
 
 class Tune(ctypes.Structure):
     _fields_ = [("i_per_lane", ctypes.c_int), ("waves_per_block", ctypes.c_int),
-                ("j_split", ctypes.c_int), ("j_tile", ctypes.c_int)]
+                ("j_split", ctypes.c_int), ("j_tile", ctypes.c_int), ("symmetric", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
