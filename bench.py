@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--quick", action="store_true", help="headline only (no extras)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=2)
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-lambda", type=float, default=4.0, help="arrivals per second")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 512 MiB device copy first (counter calibration under rocprof)")
     return ap.parse_args()
@@ -223,6 +225,114 @@ def bench_batched(ctx, steps):
             "plain": "khrr_accel per system, default launch shape, one stream"}
 
 
+def serving_requests(n_req, seed=7, tmax_lo=0.05, tmax_hi=0.5, dt=0.002):
+    """Mixed request stream: 90 % Plummer spheres (N log-uniform on
+    [1024, 131072], multiple of 1024, eps 0.004 (N/16384)^-1/3), 10 % disk +
+    halo galaxies (65536 + 131072); tmax uniform on [tmax_lo, tmax_hi]."""
+    import numpy as np
+    import ics
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for r in range(n_req):
+        if rng.uniform() < 0.1:
+            ps, vs, pd, vd = ics.galaxy(65536, 131072, seed=200000 + r)
+            pos, vel, mass, _ = ics.system_arrays(ps, vs, pd, vd, 1.0, 6.699356, 0.004, 0.004)
+            eps = np.full(pos.shape[1], 0.004)
+        else:
+            n = int(np.clip(round(np.exp(rng.uniform(np.log(1024), np.log(131072))) / 1024) * 1024,
+                            1024, 131072))
+            pos, vel = ics.plummer(n, 1.0, 0.3333, 1.0, seed=200000 + r)
+            mass = np.full(n, 1.0 / n)
+            eps = np.full(n, 0.004 * (n / 16384.0) ** (-1.0 / 3.0))
+        steps = int(round(rng.uniform(tmax_lo, tmax_hi) / dt))
+        reqs.append(dict(pos=pos, vel=vel, mass=mass, eps=eps, steps=steps, n=pos.shape[1]))
+    return reqs
+
+
+def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64):
+    """Continuous batching on a Poisson arrival stream vs the same requests
+    run one at a time (FIFO) with plain per-system launches."""
+    import numpy as np
+    import torch
+    import khrr
+    reqs = serving_requests(n_req)
+    rng = np.random.default_rng(8)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req))
+    arrivals -= arrivals[0]
+    grid = khrr.cube_grid(128, -2.0, 2.0)
+    cells = 128 ** 3
+    pairs = sum(r["n"] ** 2 * (r["steps"] + 3) for r in reqs)  # init acc+phi, steps, final phi
+
+    # served: real-time arrivals
+    srv = khrr.Server(ctx, int(budget_gb * 1e9), max_active)
+    ids, lat = {}, {}
+    busy = 0.0
+    t0 = time.perf_counter()
+    nxt = 0
+    while len(lat) < n_req:
+        now = time.perf_counter() - t0
+        while nxt < n_req and arrivals[nxt] <= now:
+            r = reqs[nxt]
+            ids[srv.submit(r["pos"], r["vel"], r["mass"], r["eps"], 0.002, r["steps"], grid)] = nxt
+            nxt += 1
+        st = srv.stats()
+        if st["active"] == 0 and st["pending"] == 0:
+            time.sleep(max(0.0, min(0.01, arrivals[nxt] - now)) if nxt < n_req else 0.0)
+            continue
+        ts = time.perf_counter()
+        finished = srv.step()
+        busy += time.perf_counter() - ts
+        for fid in finished:
+            k = ids[fid]
+            lat[k] = time.perf_counter() - t0 - arrivals[k]
+            srv.result(fid, cells)
+    makespan = time.perf_counter() - t0
+    served_pairs = srv.stats()["pair_interactions"]
+    srv.close()
+    lat_s = np.array([lat[k] for k in range(n_req)])
+    log(f"served {n_req} requests in {makespan:.2f} s, p50 {np.percentile(lat_s, 50):.3f} s, "
+        f"p99 {np.percentile(lat_s, 99):.3f} s")
+
+    # plain: each request alone, one khrr_accel per force evaluation
+    dev = torch.device("cuda:0")
+    service = []
+    for r in reqs:
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        sim = Sim(ctx, r["pos"], r["vel"], r["mass"], 0.5 * r["eps"] ** 2, 0.002)
+        sim.init_forces()
+        sim.diagnostics()
+        for _ in range(r["steps"]):
+            sim.step()
+        sim.diagnostics()
+        cnt = torch.zeros(cells, device=dev, dtype=torch.int32)
+        cm = torch.zeros(cells, device=dev, dtype=torch.float64)
+        ctx.p2mesh_bin(grid, sim.pos, sim.mass64, count=cnt, cell_mass=cm)
+        cm.cpu()
+        torch.cuda.synchronize()
+        service.append(time.perf_counter() - ts)
+        del sim
+    service = np.array(service)
+    done = np.zeros(n_req)
+    for k in range(n_req):  # FIFO queue with the same arrivals
+        done[k] = max(arrivals[k], done[k - 1] if k else 0.0) + service[k]
+    plain_lat = done - arrivals
+    log(f"plain FIFO: busy {service.sum():.2f} s, p99 {np.percentile(plain_lat, 99):.3f} s")
+    return {"requests": n_req, "lambda_per_s": lam, "tmax_range": [0.05, 0.5], "dt": 0.002,
+            "grid": "128^3 over [-2,2]^3", "budget_GB": budget_gb, "max_active": max_active,
+            "mix": "90% Plummer N in [1024,131072] log-uniform, 10% galaxies 65536+131072",
+            "pair_interactions": pairs, "served_pair_interactions": served_pairs,
+            "served_makespan_s": makespan, "served_busy_s": busy,
+            "served_Gpairs_per_s": pairs / busy / 1e9,
+            "p50_latency_s": float(np.percentile(lat_s, 50)),
+            "p99_latency_s": float(np.percentile(lat_s, 99)),
+            "plain_busy_s": float(service.sum()),
+            "plain_Gpairs_per_s": pairs / service.sum() / 1e9,
+            "plain_p50_latency_s": float(np.percentile(plain_lat, 50)),
+            "plain_p99_latency_s": float(np.percentile(plain_lat, 99)),
+            "throughput_vs_plain": float(service.sum() / busy)}
+
+
 def cpu_baseline(sim, acc_gpu, rows_n):
     """FP64 oracle on a bounded sample of i-rows at the final positions."""
     import numpy as np
@@ -312,6 +422,14 @@ def main():
     value = pairs / (ms_per_step * 1e-3) / 1e9
     grav_avg = grav_ms / max(grav_launches, 1)
     achieved = FLOP_PER_PAIR * pairs / (grav_avg * 1e-3) / 1e12
+    if tune.symmetric:
+        # off-diagonal superblock pairs once, diagonal superblocks one-sided
+        sb = 4096
+        sizes = [min(sb, n - a) for a in range(0, n, sb)]
+        diag_pairs = float(sum(s * s for s in sizes))
+        executed = (pairs - diag_pairs) / 2 + diag_pairs
+    else:
+        executed = pairs
     log(f"{ms_per_step:.2f} ms/step, gravity {grav_avg:.2f} ms/launch, {value:.1f} Gpairs/s")
     d_end = sim.diagnostics()
     drift = {"steps": args.steps,
@@ -336,9 +454,16 @@ def main():
                    "launch_shape": tune.as_dict()},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": traffic, "kernel": "grav_kernel",
+                     "traffic": traffic,
+                     "kernel": "sym_kernel" if tune.symmetric else "grav_kernel",
                      "avg_launch_ms": grav_avg, "flop_per_pair": FLOP_PER_PAIR,
-                     "pairs_per_launch": pairs},
+                     "pairs_per_launch": pairs,
+                     "executed_pair_evaluations_per_launch": executed,
+                     "note": ("pair-interactions counted as N^2 directed interactions per "
+                              "force evaluation (the reference's i = j included); the "
+                              "pair-symmetric kernel evaluates each unordered pair once and "
+                              "applies it to both bodies, so it executes "
+                              f"{executed / pairs:.4f} N^2 pair evaluations")},
         "integrator_ms_per_launch": int_ms / max(int_launches, 1),
         "drift": drift,
     }
@@ -363,6 +488,10 @@ def main():
         torch.cuda.empty_cache()
         log("batched")
         result["batched"] = bench_batched(ctx, args.batched_steps)
+        torch.cuda.empty_cache()
+        if args.serve_requests > 0:
+            log("serving")
+            result["serving"] = bench_serving(ctx, args.serve_requests, args.serve_lambda)
     print(json.dumps(result), flush=True)
 
 

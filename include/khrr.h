@@ -103,7 +103,7 @@ typedef struct {
                        evaluate each unordered pair once and apply it to
                        both bodies (needs i_per_lane 4 or 8 and a partial
                        buffer of 16 B per body per 4096 bodies, at most
-                       6 GiB; otherwise the one-sided kernel runs)       */
+                       40 GiB; otherwise the one-sided kernel runs)      */
 } khrr_tune;
 
 /* Softened direct sum.  For every i in [0,n_i):
@@ -119,13 +119,17 @@ int khrr_accel(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int
                const float* pos_j, const float* m_j, const float* h_j, float* acc, float* phi,
                unsigned flags, const khrr_tune* tune);
 
-/* Batched form: n_sys independent systems stored back to back.  sys_off is
- * a DEVICE int array of n_sys+1 prefix offsets (system s owns particles
- * [sys_off[s], sys_off[s+1])); every system interacts only with itself.
- * pos/acc are vec3 planes of n_total = sys_off[n_sys] entries.  The work
- * list (system, i-block, j-range) is built on the host from host_sys_off
- * (a HOST copy of the same offsets) in longest-first order and all systems
- * run in one persistent launch. */
+/* Batched form: n_sys independent systems stored back to back.  System s
+ * owns particles [off[s], off[s+1]) of the n_total = off[n_sys] bodies and
+ * interacts only with itself.  host_sys_off is the HOST array of the n_sys+1
+ * prefix offsets; sys_off, a device copy of it, is accepted for interface
+ * symmetry and may be NULL (the work list is built on the host).  pos/acc
+ * are vec3 planes of n_total entries.  A system of n bodies has its j-range
+ * cut into k = min(16, ceil(n / 16384)) pieces (a rule of n alone).  The
+ * work list (system, i-group, j-piece) runs longest system first in one
+ * persistent launch, and the pieces of cut systems are then summed in piece
+ * order; a system's result is therefore bit-identical to khrr_accel with
+ * j_split k, j_tile 256, symmetric 0, whatever else is in the batch. */
 int khrr_accel_batched(khrr_ctx* ctx, int n_sys, const int* sys_off, const int* host_sys_off,
                        const float* pos, const float* h, const float* m, float* acc, float* phi,
                        unsigned flags, const khrr_tune* tune);
@@ -137,7 +141,8 @@ int khrr_accel_batched(khrr_ctx* ctx, int n_sys, const int* sys_off, const int* 
  * heuristic when the bucket was never tuned). */
 int khrr_tune_gravity(khrr_ctx* ctx, int n, khrr_tune* best, double* best_ms);
 /* Same for the batched form on the composition host_sys_off[0..n_sys]
- * (i_per_lane, waves_per_block; j_split is always 1 there).  The result is
+ * (i_per_lane, waves_per_block; j_split follows the per-system rule
+ * there).  The result is
  * what khrr_accel_batched uses when its tune argument is NULL. */
 int khrr_tune_batched(khrr_ctx* ctx, int n_sys, const int* host_sys_off, khrr_tune* best,
                       double* best_ms);
@@ -253,6 +258,57 @@ int khrr_wave_step(khrr_ctx* ctx, const khrr_wave* w, float* phi_new, const floa
  * so afterwards *curr holds the newest field and *prev the one before. */
 int khrr_wave_iterate(khrr_ctx* ctx, const khrr_wave* w, float** prev, float** curr,
                       float** next, const float* src, int iters);
+
+/* ---- serving form: continuous batching ------------------------------- */
+/* No counterpart in the reference (one simulation per process).  A server
+ * keeps every admitted simulation's state resident in device memory and
+ * advances all of them together, one predictor -> gravity -> corrector
+ * step per khrr_server_step, with a single batched gravity launch over
+ * every active system.  Requests that arrive while others run are admitted
+ * (first come, first served) as soon as device memory within the budget
+ * and an active slot are free.  A finished simulation returns its
+ * conservation-law diagnostics at t = 0 and at the end, and the FP64 mass
+ * of its bodies binned on the requested grid.  Every system's arithmetic
+ * depends only on its own bodies, so a result is bit-identical whether the
+ * request ran alone or among others. */
+typedef struct khrr_server khrr_server;
+typedef struct {
+  int n;                /* bodies                                            */
+  const double* pos;    /* HOST vec3 planes of n doubles                     */
+  const double* vel;    /* HOST vec3 planes of n doubles                     */
+  const double* mass;   /* HOST n doubles                                    */
+  const double* eps;    /* HOST n softening lengths (h = eps^2 / 2)          */
+  double dt;            /* step                                              */
+  int steps;            /* predictor-corrector steps to run (tmax / dt)      */
+  khrr_grid grid;       /* grid of the returned density                     */
+} khrr_request;
+typedef struct {
+  int active;           /* simulations on the device                        */
+  int pending;          /* submitted, waiting for admission                 */
+  int finished;         /* finished, result not yet collected               */
+  long long rounds;     /* khrr_server_step calls that ran a step           */
+  double pair_interactions; /* N^2 per system per force evaluation, total   */
+  size_t bytes_in_use;  /* device bytes held by admitted simulations        */
+  size_t bytes_budget;
+} khrr_server_info;
+/* hbm_budget: device bytes the server may hold for simulation state
+ * (152 B per body); max_active: simultaneous simulations (>= 1). */
+int khrr_server_create(khrr_ctx* ctx, size_t hbm_budget, int max_active, khrr_server** out);
+void khrr_server_destroy(khrr_server* srv);
+/* Copies the request's host arrays and queues it; *id identifies it.
+ * Rejected (KHRR_ENOMEM) when it can never fit the budget. */
+int khrr_server_submit(khrr_server* srv, const khrr_request* req, long long* id);
+/* One scheduling round: admit what fits, advance every active simulation
+ * by one step, finish those that reached their step count.  Writes up to
+ * max_done finished ids to done_ids (host) and their number to *n_done.
+ * Synchronous. */
+int khrr_server_step(khrr_server* srv, long long* done_ids, int max_done, int* n_done);
+/* Copies a finished simulation's results and releases it.  diag0/diag1:
+ * KHRR_DIAG_COUNT doubles each (t = 0 and final; EP uses the potential);
+ * cell_mass: HOST array of grid.size[0]*size[1]*size[2] doubles or NULL. */
+int khrr_server_result(khrr_server* srv, long long id, double* diag0, double* diag1,
+                       double* cell_mass, size_t cell_mass_len);
+int khrr_server_stats(khrr_server* srv, khrr_server_info* out);
 
 #ifdef __cplusplus
 }

@@ -20,93 +20,163 @@ namespace {
 
 using khrr::Accum;
 
+// One work item: i-group `group` of a system whose bodies are
+// [begin, begin + count) of planes with stride `ld`, against the system's
+// j-range [j0, j1); its first j-tile's flag is tile_flags[flag_base].  A
+// system whose j-range is cut into k > 1 pieces writes partial sums for
+// piece `split` at part[NC*unit + (split*NC + c)*count + local i]
+// (split < 0: the item covers the whole j-range and writes the outputs).
+struct Item {
+  int group, begin, count, flag_base, j0, j1, unit, split;
+};
+// One cut system for the fixed-order reduction of its k partial sums.
+struct RedItem {
+  int begin, count, k, unit;
+};
+
+// Number of j-pieces of a system of n bodies: a rule of the system's own
+// size, so the summation order never depends on the rest of the batch.
+inline int pieces_of(int n) { return std::max(1, std::min(16, (n + 16383) / 16384)); }
+
 template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
 __global__ __launch_bounds__(256) void batched_kernel(
-    int n_items, const int2* __restrict__ items, const int* __restrict__ sys_off,
-    const int* __restrict__ sys_flags, int n_total, const float* __restrict__ pos,
-    const float* __restrict__ hh_all, const float* __restrict__ m, int j_tile,
-    int* __restrict__ counter, float* __restrict__ acc, float* __restrict__ phi,
-    int accumulate) {
+    int n_items, const Item* __restrict__ items, const int* __restrict__ tile_flags, int ld,
+    const float* __restrict__ pos, const float* __restrict__ hh_all, const float* __restrict__ m,
+    int j_tile, int* __restrict__ counter, float* __restrict__ acc, float* __restrict__ phi,
+    float* __restrict__ part, int accumulate) {
+  constexpr int NC = COMP ? 8 : 4;
   const int lane = threadIdx.x & (khrr::kWave - 1);
   const float* xa = pos;
-  const float* ya = pos + n_total;
-  const float* za = pos + 2 * (size_t)n_total;
+  const float* ya = pos + ld;
+  const float* za = pos + 2 * (size_t)ld;
   for (;;) {
     int got = 0;
     if (lane == 0) got = atomicAdd(counter, 1);
     const int item = __builtin_amdgcn_readfirstlane(got);
     if (item >= n_items) break;
-    const int2 it = items[item];
-    const int base = sys_off[it.x];
-    const int n = sys_off[it.x + 1] - base;
-    const int flags = sys_flags[it.x];
+    const Item it = items[item];
+    const int base = it.begin;
+    const int n = it.count;
 
     float xi[IPL], yi[IPL], zi[IPL], hi[IPL];
-    int idx[IPL];
+    int loc[IPL];
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
-      const int loc = it.y * (IPL * khrr::kWave) + k * khrr::kWave + lane;
-      idx[k] = loc < n ? base + loc : -1;
-      const bool ok = idx[k] >= 0;
-      xi[k] = ok ? xa[idx[k]] : 0.f;
-      yi[k] = ok ? ya[idx[k]] : 0.f;
-      zi[k] = ok ? za[idx[k]] : 0.f;
-      hi[k] = ok ? hh_all[idx[k]] : 1.f;
+      loc[k] = it.group * (IPL * khrr::kWave) + k * khrr::kWave + lane;
+      const bool ok = loc[k] < n;
+      const int i = base + loc[k];
+      xi[k] = ok ? xa[i] : 0.f;
+      yi[k] = ok ? ya[i] : 0.f;
+      zi[k] = ok ? za[i] : 0.f;
+      hi[k] = ok ? hh_all[i] : 1.f;
     }
     Accum<IPL> a;
     a.zero();
-    const int j1 = base + n;
-    for (int jt = base; jt < j1; jt += j_tile) {
-      const int je = min(j1, jt + j_tile);
-      khrr::tile_step<IPL, DO_ACC, DO_PHI, COMP>(jt, je, flags, xa, ya, za, m, hh_all, xi, yi, zi,
-                                                 hi, a);
+    for (int jt = it.j0, t = it.flag_base + (it.j0 - base) / j_tile; jt < it.j1;
+         jt += j_tile, ++t) {
+      const int je = min(it.j1, jt + j_tile);
+      khrr::tile_step<IPL, DO_ACC, DO_PHI, COMP>(jt, je, tile_flags[t], xa, ya, za, m, hh_all, xi,
+                                                 yi, zi, hi, a);
     }
 #pragma unroll
-    for (int k = 0; k < IPL; ++k)
-      if (idx[k] >= 0)
-        khrr::store_final<IPL, DO_ACC, DO_PHI, COMP>(a, k, idx[k], (size_t)n_total, acc, phi,
+    for (int k = 0; k < IPL; ++k) {
+      if (loc[k] >= n) continue;
+      if (it.split < 0) {
+        khrr::store_final<IPL, DO_ACC, DO_PHI, COMP>(a, k, base + loc[k], (size_t)ld, acc, phi,
                                                      accumulate);
+        continue;
+      }
+      float* p = part + (size_t)NC * it.unit + (size_t)it.split * NC * n + loc[k];
+      const size_t N = (size_t)n;
+      if (DO_ACC) {
+        p[0] = a.ax[k];
+        p[N] = a.ay[k];
+        p[2 * N] = a.az[k];
+      }
+      if (DO_PHI) p[3 * N] = a.ap[k];
+      if (COMP) {
+        if (DO_ACC) {
+          p[4 * N] = a.lx[k];
+          p[5 * N] = a.ly[k];
+          p[6 * N] = a.lz[k];
+        }
+        if (DO_PHI) p[7 * N] = a.lp[k];
+      }
+    }
   }
 }
 
-// sys_flags[s]: kUniformH / kUniformM when all h / all m of system s are
-// bitwise equal.  A system's flags depend on its own bodies only, so its
-// result does not depend on the rest of the batch.
-__global__ __launch_bounds__(256) void sys_uniform_kernel(int n_sys, const int* __restrict__ off,
-                                                          const float* __restrict__ h,
-                                                          const float* __restrict__ m,
-                                                          int* __restrict__ flags) {
-  const int lane = threadIdx.x & (khrr::kWave - 1);
-  const int s = blockIdx.x * (blockDim.x / khrr::kWave) + threadIdx.x / khrr::kWave;
-  if (s >= n_sys) return;
-  const int b = off[s], e = off[s + 1];
-  bool same_h = true, same_m = true;
-  if (e > b) {
-    const unsigned href = __float_as_uint(h[b]);
-    const unsigned mref = __float_as_uint(m[b]);
-    for (int j = b + lane; j < e; j += khrr::kWave) {
-      same_h &= __float_as_uint(h[j]) == href;
-      same_m &= __float_as_uint(m[j]) == mref;
+// Adds the k partial sums of each cut system in piece order (the order of
+// the plain kernel's split reduction) and writes the outputs.
+template <bool COMP>
+__global__ __launch_bounds__(256) void batched_reduce(const RedItem* __restrict__ red, int ld,
+                                                      int do_acc, int do_phi,
+                                                      const float* __restrict__ part,
+                                                      float* __restrict__ acc,
+                                                      float* __restrict__ phi, int accumulate) {
+  constexpr int NC = COMP ? 8 : 4;
+  const RedItem r = red[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= r.count) return;
+  const size_t N = (size_t)r.count;
+  const float* p0 = part + (size_t)NC * r.unit;
+  for (int c = 0; c < 4; ++c) {
+    if ((c < 3 && !do_acc) || (c == 3 && !do_phi)) continue;
+    float hi = p0[c * N + i];
+    float lo = COMP ? p0[(c + 4) * N + i] : 0.f;
+    for (int s = 1; s < r.k; ++s) {
+      const float* p = p0 + (size_t)s * NC * N;
+      khrr::fold(hi, lo, p[c * N + i], COMP);
+      if (COMP) lo += p[(c + 4) * N + i];
     }
+    const float v = COMP ? hi + lo : hi;
+    const size_t g = (size_t)r.begin + i;
+    if (c < 3) {
+      float* a = acc + (size_t)c * ld + g;
+      *a = accumulate ? *a + v : v;
+    } else {
+      phi[g] = accumulate ? phi[g] - v : -v;
+    }
+  }
+}
+
+// tile_flags[t]: kUniformH / kUniformM when all h / all m of j-tile t (a
+// range [tiles[t].x, tiles[t].y) inside one system) are bitwise equal.  The
+// tiles of a system start at its first body, exactly as the plain kernel's
+// tiles start at j = 0, so both forms take the same arithmetic path.
+__global__ __launch_bounds__(256) void tile_flags_kernel(int n_tiles, const int2* __restrict__ tiles,
+                                                         const float* __restrict__ h,
+                                                         const float* __restrict__ m,
+                                                         int* __restrict__ flags) {
+  const int lane = threadIdx.x & (khrr::kWave - 1);
+  const int t = blockIdx.x * (blockDim.x / khrr::kWave) + threadIdx.x / khrr::kWave;
+  if (t >= n_tiles) return;
+  const int b = tiles[t].x, e = tiles[t].y;
+  const unsigned href = __float_as_uint(h[b]);
+  const unsigned mref = __float_as_uint(m[b]);
+  bool same_h = true, same_m = true;
+  for (int j = b + lane; j < e; j += khrr::kWave) {
+    same_h &= __float_as_uint(h[j]) == href;
+    same_m &= __float_as_uint(m[j]) == mref;
   }
   const unsigned long long bad_h = __ballot(!same_h);
   const unsigned long long bad_m = __ballot(!same_m);
   if (lane == 0)
-    flags[s] = (bad_h == 0ull ? khrr::kUniformH : 0) | (bad_m == 0ull ? khrr::kUniformM : 0);
+    flags[t] = (bad_h == 0ull ? khrr::kUniformH : 0) | (bad_m == 0ull ? khrr::kUniformM : 0);
 }
 
 __global__ void set_word(int* __restrict__ p, int v) { *p = v; }
 
-using BatchLauncher = void (*)(dim3, dim3, hipStream_t, int, const int2*, const int*, const int*,
-                               int, const float*, const float*, const float*, int, int*, float*,
+using BatchLauncher = void (*)(dim3, dim3, hipStream_t, int, const Item*, const int*, int,
+                               const float*, const float*, const float*, int, int*, float*, float*,
                                float*, int);
 
 template <int IPL, bool A, bool P, bool C>
-void launch_batched(dim3 g, dim3 b, hipStream_t st, int n_items, const int2* items,
-                    const int* off, const int* uni, int n_total, const float* pos, const float* h,
-                    const float* m, int j_tile, int* counter, float* acc, float* phi, int accum) {
-  hipLaunchKernelGGL((batched_kernel<IPL, A, P, C>), g, b, 0, st, n_items, items, off, uni,
-                     n_total, pos, h, m, j_tile, counter, acc, phi, accum);
+void launch_batched(dim3 g, dim3 b, hipStream_t st, int n_items, const Item* items,
+                    const int* flags, int ld, const float* pos, const float* h, const float* m,
+                    int j_tile, int* counter, float* acc, float* phi, float* part, int accum) {
+  hipLaunchKernelGGL((batched_kernel<IPL, A, P, C>), g, b, 0, st, n_items, items, flags, ld, pos, h,
+                     m, j_tile, counter, acc, phi, part, accum);
 }
 
 template <int IPL>
@@ -126,84 +196,178 @@ BatchLauncher pick_batched(int ipl, bool a, bool p, bool c) {
   }
 }
 
-// Builds (or reuses) the device work list for this composition.
-int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_off, const int* dev_off,
-                  const float* h, const float* m, int ipl, int2** items, int** uni,
-                  int* n_items) {
-  const bool same = ctx->batch_ipl == ipl && (int)ctx->batch_off.size() == n_sys + 1 &&
-                    std::equal(ctx->batch_off.begin(), ctx->batch_off.end(), host_off);
+// Device work list of one composition.
+struct WorkList {
+  Item* items = nullptr;
+  int n_items = 0;
+  int* tile_flags = nullptr;
+  RedItem* red = nullptr;
+  int n_red = 0;
+  int max_red_count = 0;
+  long long part_units = 0;  // sum of k*count over cut systems
+};
+
+// Builds (or reuses) the device work list for this composition: items
+// longest system first, the j-tile ranges of every system, the cut
+// systems' reduction list, then the tile flags (recomputed on every call:
+// masses and softening may change).
+int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* host_count,
+                  const float* h, const float* m, int ipl, int j_tile, WorkList* wl) {
+  std::vector<int> key(2 * (size_t)n_sys + 1);
+  for (int s = 0; s < n_sys; ++s) {
+    key[2 * s] = host_begin[s];
+    key[2 * s + 1] = host_count[s];
+  }
+  key[2 * (size_t)n_sys] = j_tile;
+  const bool same = ctx->batch_ipl == ipl && ctx->batch_off == key;
   if (!same) {
+    std::vector<int> flag_base(n_sys);
+    std::vector<int2> tiles;
+    for (int s = 0; s < n_sys; ++s) {
+      flag_base[s] = (int)tiles.size();
+      const int b = host_begin[s], e = host_begin[s] + host_count[s];
+      for (int jt = b; jt < e; jt += j_tile) tiles.push_back(make_int2(jt, std::min(e, jt + j_tile)));
+    }
     std::vector<int> order(n_sys);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-      return host_off[a + 1] - host_off[a] > host_off[b + 1] - host_off[b];
-    });
-    std::vector<int2> list;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return host_count[a] > host_count[b]; });
+    std::vector<Item> list;
+    std::vector<RedItem> red;
+    long long unit = 0;
+    int max_red = 0;
     for (int s : order) {
-      const int n = host_off[s + 1] - host_off[s];
+      const int n = host_count[s], b = host_begin[s];
+      if (n == 0) continue;
       const int groups = (int)khrr::ceil_div(n, 64ll * ipl);
-      for (int g = 0; g < groups; ++g) list.push_back(make_int2(s, g));
+      // pieces as khrr_accel's j_split: chunk rounded up to whole tiles
+      long long chunk = ((long long)n + pieces_of(n) - 1) / pieces_of(n);
+      chunk = (chunk + j_tile - 1) / j_tile * j_tile;
+      const int k = (int)khrr::ceil_div(n, chunk);
+      for (int piece = 0; piece < k; ++piece) {
+        const int j0 = b + (int)(piece * chunk);
+        const int j1 = std::min(b + n, (int)(j0 + chunk));
+        for (int g = 0; g < groups; ++g)
+          list.push_back(Item{g, b, n, flag_base[s], j0, j1, k > 1 ? (int)unit : 0,
+                              k > 1 ? piece : -1});
+      }
+      if (k > 1) {
+        red.push_back(RedItem{b, n, k, (int)unit});
+        unit += (long long)k * n;
+        max_red = std::max(max_red, n);
+      }
     }
-    const size_t item_bytes = khrr::Carver::need(list.size() > 0 ? list.size() : 1, sizeof(int2));
-    const size_t need = item_bytes + khrr::Carver::need(n_sys > 0 ? n_sys : 1, sizeof(int));
+    if (unit > INT32_MAX) return khrr::fail(ctx, KHRR_EINVAL, "batch too large for partial sums");
+    const size_t ib = khrr::Carver::need(list.size() > 0 ? list.size() : 1, sizeof(Item));
+    const size_t tb = khrr::Carver::need(tiles.size() > 0 ? tiles.size() : 1, sizeof(int2));
+    const size_t rb = khrr::Carver::need(red.size() > 0 ? red.size() : 1, sizeof(RedItem));
+    const size_t fb = khrr::Carver::need(tiles.size() > 0 ? tiles.size() : 1, sizeof(int));
+    const size_t host_bytes = ib + tb + rb;
     KHRR_HIP(ctx, hipStreamSynchronize(ctx->stream));  // old list/pinned copy may be in flight
-    if (need > ctx->batch_dev_bytes) {
+    if (ib + tb + rb + fb > ctx->batch_dev_bytes) {
       if (ctx->batch_dev) KHRR_HIP(ctx, hipFree(ctx->batch_dev));
       ctx->batch_dev = nullptr;
       ctx->batch_dev_bytes = 0;
-      KHRR_HIP(ctx, hipMalloc(&ctx->batch_dev, need));
-      ctx->batch_dev_bytes = need;
+      KHRR_HIP(ctx, hipMalloc(&ctx->batch_dev, ib + tb + rb + fb));
+      ctx->batch_dev_bytes = ib + tb + rb + fb;
     }
-    if (list.size() * sizeof(int2) > ctx->batch_pinned_bytes) {
+    if (host_bytes > ctx->batch_pinned_bytes) {
       if (ctx->batch_pinned) KHRR_HIP(ctx, hipHostFree(ctx->batch_pinned));
       ctx->batch_pinned = nullptr;
       ctx->batch_pinned_bytes = 0;
-      KHRR_HIP(ctx, hipHostMalloc(&ctx->batch_pinned, list.size() * sizeof(int2)));
-      ctx->batch_pinned_bytes = list.size() * sizeof(int2);
+      KHRR_HIP(ctx, hipHostMalloc(&ctx->batch_pinned, host_bytes));
+      ctx->batch_pinned_bytes = host_bytes;
     }
-    if (!list.empty()) {
-      std::memcpy(ctx->batch_pinned, list.data(), list.size() * sizeof(int2));
-      KHRR_HIP(ctx, hipMemcpyAsync(ctx->batch_dev, ctx->batch_pinned, list.size() * sizeof(int2),
-                                   hipMemcpyHostToDevice, ctx->stream));
-    }
-    ctx->batch_off.assign(host_off, host_off + n_sys + 1);
+    char* pin = static_cast<char*>(ctx->batch_pinned);
+    char* dev = static_cast<char*>(ctx->batch_dev);
+    if (!list.empty()) std::memcpy(pin, list.data(), list.size() * sizeof(Item));
+    if (!tiles.empty()) std::memcpy(pin + ib, tiles.data(), tiles.size() * sizeof(int2));
+    if (!red.empty()) std::memcpy(pin + ib + tb, red.data(), red.size() * sizeof(RedItem));
+    KHRR_HIP(ctx, hipMemcpyAsync(dev, pin, host_bytes, hipMemcpyHostToDevice, ctx->stream));
+    ctx->batch_off = key;
     ctx->batch_ipl = ipl;
     ctx->batch_items = (int)list.size();
+    ctx->batch_tiles = (int)tiles.size();
+    ctx->batch_red = (int)red.size();
+    ctx->batch_red_max = max_red;
+    ctx->batch_units = unit;
   }
-  *items = static_cast<int2*>(ctx->batch_dev);
-  *uni = reinterpret_cast<int*>(static_cast<char*>(ctx->batch_dev) +
-                                khrr::Carver::need(ctx->batch_items > 0 ? ctx->batch_items : 1,
-                                                   sizeof(int2)));
-  *n_items = ctx->batch_items;
-  // Softening and masses can change between calls with the same composition.
-  if (n_sys > 0)
-    hipLaunchKernelGGL(sys_uniform_kernel, dim3(khrr::ceil_div(n_sys, 4)), dim3(256), 0,
-                       ctx->stream, n_sys, dev_off, h, m, *uni);
-  return khrr::check_launch(ctx, "sys_uniform_kernel");
+  const size_t ib = khrr::Carver::need(ctx->batch_items > 0 ? ctx->batch_items : 1, sizeof(Item));
+  const size_t tb = khrr::Carver::need(ctx->batch_tiles > 0 ? ctx->batch_tiles : 1, sizeof(int2));
+  const size_t rb = khrr::Carver::need(ctx->batch_red > 0 ? ctx->batch_red : 1, sizeof(RedItem));
+  char* dev = static_cast<char*>(ctx->batch_dev);
+  wl->items = reinterpret_cast<Item*>(dev);
+  const int2* tiles = reinterpret_cast<const int2*>(dev + ib);
+  wl->red = reinterpret_cast<RedItem*>(dev + ib + tb);
+  wl->tile_flags = reinterpret_cast<int*>(dev + ib + tb + rb);
+  wl->n_items = ctx->batch_items;
+  wl->n_red = ctx->batch_red;
+  wl->max_red_count = ctx->batch_red_max;
+  wl->part_units = ctx->batch_units;
+  if (ctx->batch_tiles > 0)
+    hipLaunchKernelGGL(tile_flags_kernel, dim3(khrr::ceil_div(ctx->batch_tiles, 4)), dim3(256), 0,
+                       ctx->stream, ctx->batch_tiles, tiles, h, m, wl->tile_flags);
+  return khrr::check_launch(ctx, "tile_flags_kernel");
 }
 
-int batched_launch(khrr_ctx* ctx, int n_sys, const int* sys_off, const int* host_sys_off,
-                   const float* pos, const float* h, const float* m, float* acc, float* phi,
-                   unsigned flags, const khrr_tune& t) {
-  const int n_total = host_sys_off[n_sys];
-  if (n_total == 0 || n_sys == 0) return 0;
-  int2* items = nullptr;
-  int* uni = nullptr;
-  int n_items = 0;
-  KHRR_TRY(prepare_items(ctx, n_sys, host_sys_off, sys_off, h, m, t.i_per_lane, &items, &uni,
-                         &n_items));
-  BatchLauncher fn = pick_batched(t.i_per_lane, acc != nullptr, phi != nullptr,
-                                  (flags & KHRR_COMPENSATED) != 0);
-  if (!fn) return khrr::fail(ctx, KHRR_EINVAL, "unsupported i_per_lane %d", t.i_per_lane);
+}  // namespace
+
+namespace khrr {
+
+int batched_segments(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* host_count,
+                     int ld, const float* pos, const float* h, const float* m, float* acc,
+                     float* phi, unsigned flags, const khrr_tune& t) {
+  long long total = 0;
+  for (int s = 0; s < n_sys; ++s) total += host_count[s];
+  if (total == 0 || n_sys == 0) return 0;
+  WorkList wl;
+  KHRR_TRY(prepare_items(ctx, n_sys, host_begin, host_count, h, m, t.i_per_lane, t.j_tile, &wl));
+  const bool comp = (flags & KHRR_COMPENSATED) != 0;
+  BatchLauncher fn = pick_batched(t.i_per_lane, acc != nullptr, phi != nullptr, comp);
+  if (!fn) return fail(ctx, KHRR_EINVAL, "unsupported i_per_lane %d", t.i_per_lane);
+  float* part = nullptr;
+  if (wl.n_red > 0) {
+    void* base = nullptr;
+    KHRR_TRY(scratch(ctx, Carver::need((size_t)(comp ? 8 : 4) * wl.part_units, sizeof(float)),
+                     &base));
+    part = static_cast<float*>(base);
+  }
   // Persistent grid: at most 8 waves per CU in flight, never more waves than items.
-  const long long waves = std::min<long long>(n_items, 8ll * ctx->cu_count);
+  const long long waves = std::min<long long>(wl.n_items, 8ll * ctx->cu_count);
   const int wpb = t.waves_per_block;
+  const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
   int* counter = ctx->dev_words;
-  khrr::Timed tm(ctx, KHRR_K_GRAVITY);
+  Timed tm(ctx, KHRR_K_GRAVITY);
   hipLaunchKernelGGL(set_word, dim3(1), dim3(1), 0, ctx->stream, counter, 0);
-  fn(dim3(khrr::ceil_div(waves, wpb)), dim3(64 * wpb), ctx->stream, n_items, items, sys_off, uni,
-     n_total, pos, h, m, t.j_tile, counter, acc, phi, (flags & KHRR_ACCUMULATE) ? 1 : 0);
-  return khrr::check_launch(ctx, "batched_kernel");
+  fn(dim3(ceil_div(waves, wpb)), dim3(64 * wpb), ctx->stream, wl.n_items, wl.items, wl.tile_flags,
+     ld, pos, h, m, t.j_tile, counter, acc, phi, part, accumulate);
+  if (wl.n_red > 0) {
+    const dim3 grid(ceil_div(wl.max_red_count, 256), wl.n_red);
+    if (comp)
+      hipLaunchKernelGGL(batched_reduce<true>, grid, dim3(256), 0, ctx->stream, wl.red, ld,
+                         (int)(acc != nullptr), (int)(phi != nullptr), part, acc, phi, accumulate);
+    else
+      hipLaunchKernelGGL(batched_reduce<false>, grid, dim3(256), 0, ctx->stream, wl.red, ld,
+                         (int)(acc != nullptr), (int)(phi != nullptr), part, acc, phi, accumulate);
+  }
+  return check_launch(ctx, "batched_kernel");
+}
+
+}  // namespace khrr
+
+namespace {
+
+int batched_launch(khrr_ctx* ctx, int n_sys, const int* host_sys_off, const float* pos,
+                   const float* h, const float* m, float* acc, float* phi, unsigned flags,
+                   const khrr_tune& t) {
+  std::vector<int> begin(n_sys), count(n_sys);
+  for (int s = 0; s < n_sys; ++s) {
+    begin[s] = host_sys_off[s];
+    count[s] = host_sys_off[s + 1] - host_sys_off[s];
+  }
+  return khrr::batched_segments(ctx, n_sys, begin.data(), count.data(),
+                                n_sys > 0 ? host_sys_off[n_sys] : 0, pos, h, m, acc, phi, flags,
+                                t);
 }
 
 int check_composition(khrr_ctx* ctx, int n_sys, const int* host_off) {
@@ -265,7 +429,7 @@ int khrr_accel_batched(khrr_ctx* ctx, int n_sys, const int* sys_off, const int* 
   if (flags & ~(KHRR_ACCUMULATE | KHRR_COMPENSATED))
     return khrr::fail(ctx, KHRR_EINVAL, "unknown flags 0x%x", flags);
   if (!acc && !phi) return khrr::fail(ctx, KHRR_EINVAL, "acc and phi are both NULL");
-  if (n_sys > 0 && host_sys_off[n_sys] > 0 && (!sys_off || !pos || !h || !m))
+  if (n_sys > 0 && host_sys_off[n_sys] > 0 && (!pos || !h || !m))
     return khrr::fail(ctx, KHRR_EINVAL, "null input pointer");
   khrr_tune t;
   if (tune)
@@ -276,7 +440,8 @@ int khrr_accel_batched(khrr_ctx* ctx, int n_sys, const int* sys_off, const int* 
     t = khrr_tune{4, 4, 1, 256};
   t.j_split = 1;
   KHRR_TRY(khrr::validate_tune(ctx, t));
-  return batched_launch(ctx, n_sys, sys_off, host_sys_off, pos, h, m, acc, phi, flags, t);
+  (void)sys_off;  // the host offsets define the segments
+  return batched_launch(ctx, n_sys, host_sys_off, pos, h, m, acc, phi, flags, t);
 }
 
 int khrr_tune_gravity(khrr_ctx* ctx, int n, khrr_tune* best, double* best_ms) {
@@ -384,7 +549,7 @@ int khrr_tune_batched(khrr_ctx* ctx, int n_sys, const int* host_sys_off, khrr_tu
       khrr_tune c{ipl, wpb, 1, 256};
       double ms = 0.0;
       rc = time_candidate(ctx, [&] {
-        return batched_launch(ctx, n_sys, off, host_sys_off, pos, h, m, acc, nullptr, 0, c);
+        return batched_launch(ctx, n_sys, host_sys_off, pos, h, m, acc, nullptr, 0, c);
       }, &ms);
       if (!rc && ms < best_t) {
         best_t = ms;

@@ -43,6 +43,7 @@ int scratch(khrr_ctx* ctx, size_t bytes, void** base) {
     }
     size_t want = bytes + bytes / 4 + (1u << 20);
     if (hipMalloc(&ctx->scratch, want) != hipSuccess) {
+      (void)hipGetLastError();  // clear it so a later launch check does not report it
       ctx->scratch = nullptr;
       return fail(ctx, KHRR_ENOMEM, "scratch allocation of %zu bytes failed", want);
     }

@@ -270,8 +270,11 @@ int accel_launch(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, i
     return 0;
   }
   if (tune.symmetric && n_i == n_j && pos_i == pos_j && h_i == h_j &&
-      sym_eligible(n_i, tune.i_per_lane))
-    return sym_launch(ctx, n_i, pos_i, h_i, m_j, acc, phi, flags, tune.i_per_lane);
+      sym_eligible(n_i, tune.i_per_lane)) {
+    const int rc = sym_launch(ctx, n_i, pos_i, h_i, m_j, acc, phi, flags, tune.i_per_lane);
+    if (rc != KHRR_ENOMEM) return rc;
+    // no room for the partial slots: the one-sided kernel needs none
+  }
   const int ipl = tune.i_per_lane;
   const int groups_i = (int)ceil_div(n_i, 64ll * ipl);
   const int j_tile = tune.j_tile;
@@ -343,8 +346,8 @@ int khrr_accel(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int
   } else {
     t = khrr::default_tune(ctx, n_i, n_j);
     const bool same_set = n_i == n_j && pos_i == pos_j && h_i == h_j;
-    if (same_set && khrr::sym_eligible(n_i, 4)) {
-      t.i_per_lane = 4;
+    if (same_set && khrr::sym_eligible(n_i, 8) && khrr::sym_worthwhile(ctx, n_i)) {
+      t.i_per_lane = 8;
       t.symmetric = 1;
     }
   }

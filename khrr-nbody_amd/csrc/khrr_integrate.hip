@@ -50,15 +50,17 @@ template <bool EXT>
 __global__ __launch_bounds__(256) void integrate_kernel(
     int n, int istep, double dt, ExtPot ext, const float* __restrict__ acc,
     double* __restrict__ pos_t, double* __restrict__ vel_t, const double* __restrict__ pos,
-    const double* __restrict__ vel, float* __restrict__ acc0, float* __restrict__ pos32) {
+    const double* __restrict__ vel, float* __restrict__ acc0, float* __restrict__ pos32,
+    int ld32) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const size_t N = (size_t)n;
+  const size_t L = (size_t)ld32;  // plane stride of the FP32 arrays
   const double rx = pos[i], ry = pos[N + i], rz = pos[2 * N + i];
   const double vx = vel[i], vy = vel[N + i], vz = vel[2 * N + i];
   double ex = 0.0, ey = 0.0, ez = 0.0;
   if (EXT) dev_fex(ext, rx, ry, rz, ex, ey, ez);
-  const float a_x = acc[i], a_y = acc[N + i], a_z = acc[2 * N + i];
+  const float a_x = acc[i], a_y = acc[L + i], a_z = acc[2 * L + i];
   const double fx = (double)a_x + ex;
   const double fy = (double)a_y + ey;
   const double fz = (double)a_z + ez;
@@ -77,8 +79,8 @@ __global__ __launch_bounds__(256) void integrate_kernel(
     vel_t[2 * N + i] = tz;
     if (pos32) {
       pos32[i] = (float)px;
-      pos32[N + i] = (float)py;
-      pos32[2 * N + i] = (float)pz;
+      pos32[L + i] = (float)py;
+      pos32[2 * L + i] = (float)pz;
     }
   } else {
     const double tx = vel_t[i], ty = vel_t[N + i], tz = vel_t[2 * N + i];
@@ -93,13 +95,13 @@ __global__ __launch_bounds__(256) void integrate_kernel(
     vel_t[2 * N + i] = uz;
     if (acc0) {
       acc0[i] = a_x;
-      acc0[N + i] = a_y;
-      acc0[2 * N + i] = a_z;
+      acc0[L + i] = a_y;
+      acc0[2 * L + i] = a_z;
     }
     if (pos32) {
       pos32[i] = (float)rx;
-      pos32[N + i] = (float)ry;
-      pos32[2 * N + i] = (float)rz;
+      pos32[L + i] = (float)ry;
+      pos32[2 * L + i] = (float)rz;
     }
   }
 }
@@ -110,7 +112,7 @@ namespace khrr {
 
 int integrate_launch(khrr_ctx* ctx, int n, int istep, double dt, const khrr_extpot* ext,
                      const float* acc, double* pos_t, double* vel_t, const double* pos,
-                     const double* vel, float* acc0, float* pos32) {
+                     const double* vel, float* acc0, float* pos32, int ld32) {
   if (n == 0) return 0;
   ExtPot e{};
   const bool has_ext = ext && (ext->Mh != 0.0 || ext->Mb != 0.0);
@@ -128,10 +130,10 @@ int integrate_launch(khrr_ctx* ctx, int n, int istep, double dt, const khrr_extp
   const dim3 grid(ceil_div(n, 256)), block(256);
   if (has_ext)
     hipLaunchKernelGGL(integrate_kernel<true>, grid, block, 0, ctx->stream, n, istep, dt, e, acc,
-                       pos_t, vel_t, pos, vel, acc0, pos32);
+                       pos_t, vel_t, pos, vel, acc0, pos32, ld32);
   else
     hipLaunchKernelGGL(integrate_kernel<false>, grid, block, 0, ctx->stream, n, istep, dt, e, acc,
-                       pos_t, vel_t, pos, vel, acc0, pos32);
+                       pos_t, vel_t, pos, vel, acc0, pos32, ld32);
   return check_launch(ctx, "integrate_kernel");
 }
 
@@ -196,7 +198,8 @@ int khrr_integrate(khrr_ctx* ctx, int n, int istep, double dt, double t, const k
   if (istep != 0 && istep != 1) return khrr::fail(ctx, KHRR_EINVAL, "istep must be 0 or 1");
   if (n > 0 && (!acc || !pos_t || !vel_t || !pos || !vel))
     return khrr::fail(ctx, KHRR_EINVAL, "null state pointer");
-  return khrr::integrate_launch(ctx, n, istep, dt, ext, acc, pos_t, vel_t, pos, vel, acc0, pos32);
+  return khrr::integrate_launch(ctx, n, istep, dt, ext, acc, pos_t, vel_t, pos, vel, acc0, pos32,
+                                n);
 }
 
 }  // extern "C"

@@ -54,7 +54,11 @@ struct khrr_ctx {
   std::vector<int> batch_off;
   int batch_ipl = 0;
   int batch_items = 0;
-  void* batch_dev = nullptr;  // items (int2) then per-system uniform flags
+  int batch_tiles = 0;
+  int batch_red = 0;          // systems whose j-range is cut into pieces
+  int batch_red_max = 0;      // largest such system
+  long long batch_units = 0;  // partial-sum rows of all cut systems
+  void* batch_dev = nullptr;  // items, j-tile ranges, cut systems, tile flags
   size_t batch_dev_bytes = 0;
   void* batch_pinned = nullptr;
   size_t batch_pinned_bytes = 0;
@@ -115,8 +119,19 @@ void store_tune(khrr_ctx* ctx, int n, const khrr_tune& t);
 int accel_launch(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int n_j,
                  const float* pos_j, const float* m_j, const float* h_j, float* acc, float* phi,
                  unsigned flags, const khrr_tune& tune);
+// Predictor/corrector (khrr_integrate.hip); ld32 is the plane stride of the
+// FP32 arrays acc, acc0 and pos32 (n for the public entry point).
+int integrate_launch(khrr_ctx* ctx, int n, int istep, double dt, const khrr_extpot* ext,
+                     const float* acc, double* pos_t, double* vel_t, const double* pos,
+                     const double* vel, float* acc0, float* pos32, int ld32);
+// Batched form over arbitrary segments [begin, begin+count) of planes with
+// stride ld (khrr_batch.hip); host arrays describe the segments.
+int batched_segments(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* host_count,
+                     int ld, const float* pos, const float* h, const float* m, float* acc,
+                     float* phi, unsigned flags, const khrr_tune& t);
 // Pair-symmetric form (khrr_sym.hip) for one system (i-set == j-set).
 bool sym_eligible(int n, int ipl);
+bool sym_worthwhile(const khrr_ctx* ctx, int n);
 int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const float* m, float* acc,
                float* phi, unsigned flags, int ipl);
 

@@ -279,13 +279,21 @@ SymLauncher pick_sym_mode(bool a, bool p, bool c) {
 
 namespace khrr {
 
-constexpr size_t kSymPartBudget = size_t(6) << 30;  // partial-slot buffer cap (6 GiB)
+// Partial-slot buffer cap: 16 B per body per superblock is 2.4 GB at
+// 786432 bodies and 38.6 GB at 3145728 (of 288 GB HBM).
+constexpr size_t kSymPartBudget = size_t(40) << 30;
 
 bool sym_eligible(int n, int ipl) {
   if (ipl != 4 && ipl != 8) return false;
   const long long n_sb = (n + kSB - 1) / kSB;
   if (n_sb < 2) return false;
   return (size_t)n_sb * 4 * (size_t)n * sizeof(float) <= kSymPartBudget;
+}
+
+// Enough (A, B) items to fill the device (about two workgroups per CU).
+bool sym_worthwhile(const khrr_ctx* ctx, int n) {
+  const long long n_sb = (n + kSB - 1) / kSB;
+  return n_sb * (n_sb + 1) / 2 >= 2ll * ctx->cu_count;
 }
 
 int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const float* m, float* acc,

@@ -72,6 +72,21 @@ class Wave(ctypes.Structure):
                 ("bc_l", ctypes.c_double)]
 
 
+class Request(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("pos", ctypes.c_void_p), ("vel", ctypes.c_void_p),
+                ("mass", ctypes.c_void_p), ("eps", ctypes.c_void_p), ("dt", ctypes.c_double),
+                ("steps", ctypes.c_int), ("grid", Grid)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("active", ctypes.c_int), ("pending", ctypes.c_int), ("finished", ctypes.c_int),
+                ("rounds", ctypes.c_longlong), ("pair_interactions", ctypes.c_double),
+                ("bytes_in_use", ctypes.c_size_t), ("bytes_budget", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
@@ -100,6 +115,13 @@ _SIGS = {
     "khrr_p2mesh_bin": (_I, [_P, ctypes.POINTER(Grid), _I, _P, _P, _P, _P, _P, _P, _P, _P,
                              _P]),
     "khrr_p2mesh_cell_phi": (_I, [_P, _I, ctypes.c_longlong, _P, _P, _P, _P, _P, _P, _I]),
+    "khrr_server_create": (_I, [_P, ctypes.c_size_t, _I, ctypes.POINTER(_P)]),
+    "khrr_server_destroy": (None, [_P]),
+    "khrr_server_submit": (_I, [_P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_longlong)]),
+    "khrr_server_step": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), _I, ctypes.POINTER(_I)]),
+    "khrr_server_result": (_I, [_P, ctypes.c_longlong, ctypes.POINTER(_D), ctypes.POINTER(_D),
+                                ctypes.POINTER(_D), ctypes.c_size_t]),
+    "khrr_server_stats": (_I, [_P, ctypes.POINTER(ServerStats)]),
     "khrr_wave_source": (_I, [_P, ctypes.POINTER(Wave), _P, _P]),
     "khrr_wave_step": (_I, [_P, ctypes.POINTER(Wave), _P, _P, _P, _P]),
     "khrr_wave_iterate": (_I, [_P, ctypes.POINTER(Wave), ctypes.POINTER(_P), ctypes.POINTER(_P),
@@ -378,3 +400,65 @@ def cube_grid(nx: int, lo: float, hi: float) -> Grid:
     d = (hi - lo) / nx
     return Grid((ctypes.c_double * 3)(lo, lo, lo), (ctypes.c_double * 3)(d, d, d),
                 (ctypes.c_int * 3)(nx, nx, nx))
+
+
+class Server:
+    """Continuous-batching server (khrr_server_*): admits simulations by
+    device-memory footprint, keeps their state on the device and advances
+    all of them with one batched gravity launch per step."""
+
+    def __init__(self, ctx: Context, budget_bytes: int, max_active: int):
+        self._ctx = ctx
+        self._lib = ctx._lib
+        h = _P()
+        ctx._check(self._lib.khrr_server_create(ctx._h, int(budget_bytes), int(max_active),
+                                                ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.khrr_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, pos, vel, mass, eps, dt: float, steps: int, grid: Grid) -> int:
+        """pos, vel: float64 (3, n); mass, eps: float64 (n,) host arrays."""
+        import numpy as np
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        n = pos.shape[1]
+        if pos.shape != (3, n) or vel.shape != (3, n) or mass.shape != (n,) or eps.shape != (n,):
+            raise ValueError("request arrays must be (3, n), (3, n), (n,), (n,)")
+        req = Request(n, pos.ctypes.data, vel.ctypes.data, mass.ctypes.data, eps.ctypes.data,
+                      float(dt), int(steps), grid)
+        rid = ctypes.c_longlong()
+        self._ctx._check(self._lib.khrr_server_submit(self._h, ctypes.byref(req),
+                                                      ctypes.byref(rid)))
+        return rid.value
+
+    def step(self, max_done: int = 4096):
+        ids = (ctypes.c_longlong * max_done)()
+        nd = ctypes.c_int()
+        self._ctx._check(self._lib.khrr_server_step(self._h, ids, max_done, ctypes.byref(nd)))
+        return [ids[k] for k in range(min(nd.value, max_done))]
+
+    def result(self, rid: int, cells: int):
+        import numpy as np
+        d0 = (_D * len(DIAG_NAMES))()
+        d1 = (_D * len(DIAG_NAMES))()
+        cm = np.zeros(cells, dtype=np.float64)
+        self._ctx._check(self._lib.khrr_server_result(
+            self._h, rid, d0, d1, cm.ctypes.data_as(ctypes.POINTER(_D)), cells))
+        return dict(zip(DIAG_NAMES, list(d0))), dict(zip(DIAG_NAMES, list(d1))), cm
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self._ctx._check(self._lib.khrr_server_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()

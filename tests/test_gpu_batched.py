@@ -1,7 +1,7 @@
 """The batched form (khrr_accel_batched): many independent systems of uneven N
 in one persistent launch.  Each system's result must equal the plain
-single-system launch (j_split 1, j_tile 256) bit for bit, whatever else is
-in the batch, and must match the FP64 oracle within the gravity tolerance
+single-system launch (j_split min(16, ceil(n/16384)), j_tile 256) bit for
+bit, whatever else is in the batch, and must match the FP64 oracle within the gravity tolerance
 (median <= 1e-6, p99 <= 1e-5)."""
 import numpy as np
 import pytest
@@ -38,6 +38,10 @@ def _run_batched(ctx, off, pos, m, h, tune=None, phi=False, compensated=False):
     return acc.cpu().numpy(), (ph.cpu().numpy() if phi else None)
 
 
+def _pieces(n):
+    return max(1, min(16, -(-n // 16384)))
+
+
 def _run_plain(ctx, pos, m, h, phi=False, compensated=False):
     import khrr
     import torch
@@ -47,7 +51,7 @@ def _run_plain(ctx, pos, m, h, phi=False, compensated=False):
     p = to_dev(pos, torch.float32)
     hh = to_dev(h, torch.float32)
     ctx.accel(p, hh, p, to_dev(m, torch.float32), hh, acc=acc, phi=ph,
-              tune=khrr.Tune(4, 4, 1, 256), compensated=compensated)
+              tune=khrr.Tune(4, 4, _pieces(n), 256), compensated=compensated)
     torch.cuda.synchronize()
     return acc.cpu().numpy(), (ph.cpu().numpy() if phi else None)
 
@@ -76,6 +80,21 @@ def test_batched_oracle_and_shapes(ctx, ipl):
         assert st["median"] <= 1e-6 and st["p99"] <= 1e-5, (s, st)
 
 
+def test_nonuniform_galaxy_equals_plain(ctx):
+    # stars and halo bodies of different mass in one system: both forms take
+    # the same per-tile paths
+    from helpers import galaxy_system
+    pos, _, mass, h = galaxy_system(1000, 3000, seed=9, eps_s=0.004, eps_d=0.006)
+    p32, m32, h32 = pos.astype(np.float32), mass.astype(np.float32), h.astype(np.float32)
+    off = np.array([0, 1500, p32.shape[1]], np.int32)
+    acc, phi = _run_batched(ctx, off, p32, m32, h32, phi=True)
+    for s in range(2):
+        a, b = off[s], off[s + 1]
+        pa, pp = _run_plain(ctx, p32[:, a:b].copy(), m32[a:b].copy(), h32[a:b].copy(), phi=True)
+        np.testing.assert_array_equal(acc[:, a:b], pa)
+        np.testing.assert_array_equal(phi[a:b], pp)
+
+
 def test_alone_equals_in_the_mix(ctx):
     # the same system alone and embedded among others (different position
     # in the work list, different neighbours): identical bits
@@ -95,6 +114,26 @@ def test_composition_change_and_empty_systems(ctx):
     acc1b, _ = _run_batched(ctx, off, pos, m, h)
     np.testing.assert_array_equal(acc1, acc1b)
     assert np.isfinite(acc2).all()
+
+
+@pytest.mark.parametrize("compensated", [False, True])
+def test_cut_systems_equal_plain_split(ctx, compensated):
+    # systems above 16384 bodies have their j-range cut into pieces summed
+    # in piece order; the plain launch with the same j_split agrees exactly
+    sizes = [40000, 1000, 16385, 70000, 16384]
+    off, pos, m, h = _population(sizes, seed0=700)
+    acc, phi = _run_batched(ctx, off, pos, m, h, phi=True, compensated=compensated)
+    for s in range(len(sizes)):
+        a, b = off[s], off[s + 1]
+        pa, pp = _run_plain(ctx, pos[:, a:b].copy(), m[a:b].copy(), h[a:b].copy(), phi=True,
+                            compensated=compensated)
+        np.testing.assert_array_equal(acc[:, a:b], pa, err_msg=f"system {s}")
+        np.testing.assert_array_equal(phi[a:b], pp, err_msg=f"system {s}")
+    a, b = off[0], off[1]
+    rows = np.arange(0, b - a, 97)
+    ref, _ = orc.accel_rows(rows, pos[:, a:b], h[a:b], m[a:b])
+    st = err_stats(rel_err(acc[:, a + rows], ref))
+    assert st["median"] <= 1e-6 and st["p99"] <= 1e-5, st
 
 
 def test_bad_offsets_rejected(ctx):

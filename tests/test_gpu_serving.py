@@ -1,0 +1,115 @@
+"""The serving form (khrr_server_*): continuous batching with admission by
+device-memory footprint.  Each result must be bit-identical whether the
+request ran alone or among others; energies follow the FP64 oracle loop
+(nbody.cu:1468-1594) within FP32-force tolerance (1e-5 relative)."""
+import numpy as np
+import pytest
+
+import ics
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _plummer_req(n, seed):
+    pos, vel = ics.plummer(n, 1.0, 0.3333, 1.0, seed=seed)
+    eps = 0.004 * (n / 16384.0) ** (-1.0 / 3.0)
+    return pos, vel, np.full(n, 1.0 / n), np.full(n, eps)
+
+
+def _galaxy_req(n_s, n_d, seed):
+    ps, vs, pd, vd = ics.galaxy(n_s, n_d, seed=seed)
+    pos, vel, mass, _ = ics.system_arrays(ps, vs, pd, vd, 1.0, 6.699356, 0.004, 0.004)
+    return pos, vel, mass, np.full(pos.shape[1], 0.004)
+
+
+def _grid():
+    import khrr
+    return khrr.cube_grid(16, -2.0, 2.0)
+
+
+def _run_all(srv, reqs, arrivals):
+    """Submits request k before round arrivals[k]; steps until all finish."""
+    ids = {}
+    done = {}
+    rnd = 0
+    while len(done) < len(reqs):
+        for k, r in enumerate(reqs):
+            if k not in ids and arrivals[k] <= rnd:
+                ids[k] = srv.submit(*r["arrays"], 0.002, r["steps"], _grid())
+        for fid in srv.step():
+            k = next(k for k, v in ids.items() if v == fid)
+            done[k] = srv.result(fid, 16 ** 3)
+        rnd += 1
+        assert rnd < 10000
+    return done
+
+
+def test_alone_equals_in_the_mix(ctx):
+    import khrr
+    target = dict(arrays=_galaxy_req(1024, 2048, 5), steps=12)
+    others = [dict(arrays=_plummer_req(n, 40 + n), steps=s)
+              for n, s in ((4096, 20), (1024, 5), (2048, 9), (8192, 15))]
+    alone = khrr.Server(ctx, 1 << 30, 8)
+    r_alone = _run_all(alone, [target], [0])[0]
+    alone.close()
+    mix = khrr.Server(ctx, 1 << 30, 8)
+    reqs = others[:2] + [target] + others[2:]
+    r_mix = _run_all(mix, reqs, [0, 0, 3, 4, 6])[2]
+    mix.close()
+    for a, b in zip(r_alone[:2], r_mix[:2]):
+        assert a == b  # diagnostics dicts, exact
+    np.testing.assert_array_equal(r_alone[2], r_mix[2])
+    assert abs(r_mix[2].sum() - (1.0 + 6.699356)) < 1e-9
+
+
+def test_energy_follows_oracle(ctx):
+    import khrr
+    pos, vel, mass, eps = _plummer_req(2048, 77)
+    srv = khrr.Server(ctx, 1 << 28, 4)
+    res = _run_all(srv, [dict(arrays=(pos, vel, mass, eps), steps=25)], [0])[0]
+    srv.close()
+    h = (0.5 * eps * eps).astype(np.float32).astype(np.float64)
+    m32 = mass.astype(np.float32).astype(np.float64)
+    _, _, d = orc.run(pos, vel, m32, h, 0.002, 25)
+    E = lambda dd: 0.5 * (dd["Ek"] + dd["Ep"])  # noqa: E731
+    # the server uses FP64 masses in the sums and FP32 masses in gravity
+    assert abs(E(res[0]) / (0.5 * (d[0, 0] + d[0, 1])) - 1) < 1e-5
+    assert abs(E(res[1]) / (0.5 * (d[-1, 0] + d[-1, 1])) - 1) < 1e-5
+
+
+def test_admission_respects_budget(ctx):
+    import khrr
+    # 152 B per body: a 3 MB budget holds ~20k bodies, i.e. two 8192 systems
+    srv = khrr.Server(ctx, 3_000_000, 16)
+    reqs = [dict(arrays=_plummer_req(8192, 90 + k), steps=4) for k in range(4)]
+    ids = [srv.submit(*r["arrays"], 0.002, r["steps"], _grid()) for r in reqs]
+    done = srv.step()
+    st = srv.stats()
+    assert st["active"] == 2 and st["pending"] == 2 and not done
+    assert st["bytes_in_use"] <= st["bytes_budget"]
+    finished = []
+    for _ in range(100):
+        finished += srv.step()
+        if len(finished) == 4:
+            break
+    assert sorted(finished) == sorted(ids)
+    for rid in ids:
+        d0, d1, cm = srv.result(rid, 16 ** 3)
+        assert d0["M"] == pytest.approx(1.0) and cm.sum() == pytest.approx(1.0)
+    srv.close()
+
+
+def test_errors(ctx):
+    import khrr
+    srv = khrr.Server(ctx, 1_000_000, 2)
+    big = _plummer_req(8192, 3)
+    with pytest.raises(khrr.KhrrError, match="never fit"):
+        srv.submit(*big, 0.002, 3, _grid())
+    small = _plummer_req(1024, 4)
+    rid = srv.submit(*small, 0.002, 3, _grid())
+    with pytest.raises(khrr.KhrrError, match="not finished"):
+        srv.result(rid, 16 ** 3)
+    with pytest.raises(khrr.KhrrError, match="unknown request"):
+        srv.result(12345, 16 ** 3)
+    srv.close()
