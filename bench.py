@@ -56,7 +56,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--serve-lambda", type=float, default=4.0, help="arrivals per second")
+    ap.add_argument("--serve-lambda", type=float, default=3.0, help="arrivals per second")
+    ap.add_argument("--shape", default=None,
+                    help="fixed launch shape i_per_lane,waves_per_block,j_split,j_tile,symmetric "
+                         "instead of the autotuner (for profiler passes)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 512 MiB device copy first (counter calibration under rocprof)")
     return ap.parse_args()
@@ -84,9 +87,11 @@ class Sim:
         self.acct = torch.zeros((3, n), **f32)
         self.phi = torch.zeros(n, **f32)
         self.t = 0.0
+        self.tune = None  # launch shape; None = the context's tuned/static choice
 
     def gravity(self, out, phi=None):
-        self.ctx.accel(self.pos32, self.h32, self.pos32, self.m32, self.h32, acc=out, phi=phi)
+        self.ctx.accel(self.pos32, self.h32, self.pos32, self.m32, self.h32, acc=out, phi=phi,
+                       tune=self.tune)
 
     def init_forces(self):
         self.gravity(self.acc)
@@ -101,7 +106,8 @@ class Sim:
         self.t += self.dt
 
     def diagnostics(self):
-        self.ctx.accel(self.pos32, self.h32, self.pos32, self.m32, self.h32, phi=self.phi)
+        self.ctx.accel(self.pos32, self.h32, self.pos32, self.m32, self.h32, phi=self.phi,
+                       tune=self.tune)
         return self.ctx.diag(self.pos, self.vel, self.mass64, self.phi)
 
 
@@ -204,6 +210,14 @@ def bench_batched(ctx, steps):
         for p, hh, mm, o in per_sys:
             ctx.accel(p, hh, p, mm, hh, acc=o)
 
+    # the reference's launch shape: one body per lane, no j split
+    ref_shape = khrr.Tune(1, 4, 1, 256, 0)
+
+    def grav_ref_shape(out):
+        del out
+        for p, hh, mm, o in per_sys:
+            ctx.accel(p, hh, p, mm, hh, acc=o, tune=ref_shape)
+
     def run(grav, k):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -218,11 +232,18 @@ def bench_batched(ctx, steps):
     grav_plain(sim.acct)
     t_p = run(grav_plain, max(1, steps // 2))
     log(f"plain per-system: {t_p * 1e3:.1f} ms per force evaluation")
+    grav_ref_shape(sim.acct)
+    t_r = run(grav_ref_shape, 1)
+    log(f"plain per-system, reference launch shape: {t_r * 1e3:.1f} ms per force evaluation")
     return {"systems": 512, "bodies": N, "pairs_per_step": pairs,
             "batched_ms": t_b * 1e3, "plain_ms": t_p * 1e3,
             "batched_Gpairs_per_s": pairs / t_b / 1e9, "plain_Gpairs_per_s": pairs / t_p / 1e9,
             "speedup_vs_plain": t_p / t_b, "tune": tune.as_dict(),
-            "plain": "khrr_accel per system, default launch shape, one stream"}
+            "plain": "khrr_accel per system, default launch shape, one stream",
+            "plain_reference_shape_ms": t_r * 1e3,
+            "speedup_vs_plain_reference_shape": t_r / t_b,
+            "plain_reference_shape": "khrr_accel per system with one body per lane and no "
+                                     "j split (the reference's one-thread-per-body launch)"}
 
 
 def serving_requests(n_req, seed=7, tmax_lo=0.05, tmax_hi=0.5, dt=0.002):
@@ -249,7 +270,7 @@ def serving_requests(n_req, seed=7, tmax_lo=0.05, tmax_hi=0.5, dt=0.002):
     return reqs
 
 
-def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64):
+def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4e10):
     """Continuous batching on a Poisson arrival stream vs the same requests
     run one at a time (FIFO) with plain per-system launches."""
     import numpy as np
@@ -263,35 +284,55 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64):
     cells = 128 ** 3
     pairs = sum(r["n"] ** 2 * (r["steps"] + 3) for r in reqs)  # init acc+phi, steps, final phi
 
-    # served: real-time arrivals
-    srv = khrr.Server(ctx, int(budget_gb * 1e9), max_active)
-    ids, lat = {}, {}
-    busy = 0.0
-    t0 = time.perf_counter()
-    nxt = 0
-    while len(lat) < n_req:
-        now = time.perf_counter() - t0
-        while nxt < n_req and arrivals[nxt] <= now:
-            r = reqs[nxt]
-            ids[srv.submit(r["pos"], r["vel"], r["mass"], r["eps"], 0.002, r["steps"], grid)] = nxt
-            nxt += 1
+    def served(round_pairs):
+        """Real-time arrivals into one server; returns latencies and figures."""
+        srv = khrr.Server(ctx, int(budget_gb * 1e9), max_active, round_pairs=round_pairs)
+        ctx.reset_timing()
+        ctx.set_timing(True)
+        ids, lat = {}, {}
+        busy = 0.0
+        t0 = time.perf_counter()
+        nxt = 0
+        while len(lat) < n_req:
+            now = time.perf_counter() - t0
+            while nxt < n_req and arrivals[nxt] <= now:
+                r = reqs[nxt]
+                ids[srv.submit(r["pos"], r["vel"], r["mass"], r["eps"], 0.002, r["steps"],
+                               grid)] = nxt
+                nxt += 1
+            st = srv.stats()
+            if st["active"] == 0 and st["pending"] == 0:
+                time.sleep(max(0.0, min(0.01, arrivals[nxt] - now)) if nxt < n_req else 0.0)
+                continue
+            ts = time.perf_counter()
+            finished = srv.step()
+            busy += time.perf_counter() - ts
+            for fid in finished:
+                k = ids[fid]
+                lat[k] = time.perf_counter() - t0 - arrivals[k]
+                srv.result(fid, cells)
+        makespan = time.perf_counter() - t0
         st = srv.stats()
-        if st["active"] == 0 and st["pending"] == 0:
-            time.sleep(max(0.0, min(0.01, arrivals[nxt] - now)) if nxt < n_req else 0.0)
-            continue
-        ts = time.perf_counter()
-        finished = srv.step()
-        busy += time.perf_counter() - ts
-        for fid in finished:
-            k = ids[fid]
-            lat[k] = time.perf_counter() - t0 - arrivals[k]
-            srv.result(fid, cells)
-    makespan = time.perf_counter() - t0
-    served_pairs = srv.stats()["pair_interactions"]
-    srv.close()
-    lat_s = np.array([lat[k] for k in range(n_req)])
-    log(f"served {n_req} requests in {makespan:.2f} s, p50 {np.percentile(lat_s, 50):.3f} s, "
-        f"p99 {np.percentile(lat_s, 99):.3f} s")
+        srv.close()
+        _, grav_ms = ctx.kernel_time(khrr.K_GRAVITY)
+        _, int_ms = ctx.kernel_time(khrr.K_INTEGRATE)
+        ctx.set_timing(False)
+        lat_s = np.array([lat[k] for k in range(n_req)])
+        return lat_s, {"round_pairs": round_pairs, "makespan_s": makespan, "busy_s": busy,
+                       "rounds": st["rounds"], "gravity_kernel_s": grav_ms * 1e-3,
+                       "integrator_kernel_s": int_ms * 1e-3,
+                       "pair_interactions": st["pair_interactions"],
+                       "Gpairs_per_s": pairs / busy / 1e9,
+                       "p50_latency_s": float(np.percentile(lat_s, 50)),
+                       "p99_latency_s": float(np.percentile(lat_s, 99))}
+
+    policies = {}
+    for name, rp in (("lockstep", 0.0), ("least_work_first", round_pairs)):
+        lat_s, fig = served(rp)
+        policies[name] = fig
+        log(f"served ({name}) {n_req} requests in {fig['makespan_s']:.2f} s, "
+            f"busy {fig['busy_s']:.2f} s, p50 {fig['p50_latency_s']:.3f} s, "
+            f"p99 {fig['p99_latency_s']:.3f} s")
 
     # plain: each request alone, one khrr_accel per force evaluation
     dev = torch.device("cuda:0")
@@ -318,19 +359,20 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64):
         done[k] = max(arrivals[k], done[k - 1] if k else 0.0) + service[k]
     plain_lat = done - arrivals
     log(f"plain FIFO: busy {service.sum():.2f} s, p99 {np.percentile(plain_lat, 99):.3f} s")
+    head = policies["least_work_first"]
     return {"requests": n_req, "lambda_per_s": lam, "tmax_range": [0.05, 0.5], "dt": 0.002,
             "grid": "128^3 over [-2,2]^3", "budget_GB": budget_gb, "max_active": max_active,
             "mix": "90% Plummer N in [1024,131072] log-uniform, 10% galaxies 65536+131072",
-            "pair_interactions": pairs, "served_pair_interactions": served_pairs,
-            "served_makespan_s": makespan, "served_busy_s": busy,
-            "served_Gpairs_per_s": pairs / busy / 1e9,
-            "p50_latency_s": float(np.percentile(lat_s, 50)),
-            "p99_latency_s": float(np.percentile(lat_s, 99)),
+            "pair_interactions": pairs,
+            "policy": "least_work_first",
+            "served_busy_s": head["busy_s"], "served_Gpairs_per_s": head["Gpairs_per_s"],
+            "p50_latency_s": head["p50_latency_s"], "p99_latency_s": head["p99_latency_s"],
+            "policies": policies,
             "plain_busy_s": float(service.sum()),
             "plain_Gpairs_per_s": pairs / service.sum() / 1e9,
             "plain_p50_latency_s": float(np.percentile(plain_lat, 50)),
             "plain_p99_latency_s": float(np.percentile(plain_lat, 99)),
-            "throughput_vs_plain": float(service.sum() / busy)}
+            "throughput_vs_plain": float(service.sum() / head["busy_s"])}
 
 
 def cpu_baseline(sim, acc_gpu, rows_n):
@@ -394,9 +436,14 @@ def main():
     ctx = khrr.Context(0)
     # launch shape from the project's autotuner (cached in the context for
     # this size bucket; every later khrr_accel of this size uses it)
-    tune, tune_ms = ctx.tune_gravity(n)
-    log(f"autotuned launch shape {tune.as_dict()} ({tune_ms:.2f} ms per trial)")
+    if args.shape:
+        tune = khrr.Tune(*[int(v) for v in args.shape.split(",")])
+        log(f"fixed launch shape {tune.as_dict()}")
+    else:
+        tune, tune_ms = ctx.tune_gravity(n)
+        log(f"autotuned launch shape {tune.as_dict()} ({tune_ms:.2f} ms per trial)")
     sim = Sim(ctx, pos, vel, mass, h, 0.002)
+    sim.tune = tune
     sim.init_forces()
     torch.cuda.synchronize()
     log(f"n={n}; warmup {args.warmup} steps")

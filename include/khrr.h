@@ -124,12 +124,16 @@ int khrr_accel(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int
  * interacts only with itself.  host_sys_off is the HOST array of the n_sys+1
  * prefix offsets; sys_off, a device copy of it, is accepted for interface
  * symmetry and may be NULL (the work list is built on the host).  pos/acc
- * are vec3 planes of n_total entries.  A system of n bodies has its j-range
- * cut into k = min(16, ceil(n / 16384)) pieces (a rule of n alone).  The
- * work list (system, i-group, j-piece) runs longest system first in one
- * persistent launch, and the pieces of cut systems are then summed in piece
- * order; a system's result is therefore bit-identical to khrr_accel with
- * j_split k, j_tile 256, symmetric 0, whatever else is in the batch. */
+ * are vec3 planes of n_total entries.  Per-system rules, functions of the
+ * system's own n only:
+ *  - n >= 32768 (when its partial slots fit, as for khrr_accel): the
+ *    pair-symmetric path, one workgroup per (A, B) superblock item; the
+ *    result is bit-identical to khrr_accel with i_per_lane 8, symmetric 1;
+ *  - otherwise the j-range is cut into k = min(16, ceil(n / 16384)) pieces
+ *    and (i-group, j-piece) items run longest system first in one
+ *    persistent launch, pieces summed in piece order; the result is
+ *    bit-identical to khrr_accel with j_split k, j_tile 256, symmetric 0.
+ * So a system's result never depends on what else is in the batch. */
 int khrr_accel_batched(khrr_ctx* ctx, int n_sys, const int* sys_off, const int* host_sys_off,
                        const float* pos, const float* h, const float* m, float* acc, float* phi,
                        unsigned flags, const khrr_tune* tune);
@@ -264,7 +268,8 @@ int khrr_wave_iterate(khrr_ctx* ctx, const khrr_wave* w, float** prev, float** c
  * keeps every admitted simulation's state resident in device memory and
  * advances all of them together, one predictor -> gravity -> corrector
  * step per khrr_server_step, with a single batched gravity launch over
- * every active system.  Requests that arrive while others run are admitted
+ * every active system (khrr_accel_batched's per-system rules).  Requests
+ * that arrive while others run are admitted
  * (first come, first served) as soon as device memory within the budget
  * and an active slot are free.  A finished simulation returns its
  * conservation-law diagnostics at t = 0 and at the end, and the FP64 mass
@@ -291,10 +296,21 @@ typedef struct {
   size_t bytes_in_use;  /* device bytes held by admitted simulations        */
   size_t bytes_budget;
 } khrr_server_info;
-/* hbm_budget: device bytes the server may hold for simulation state
- * (152 B per body); max_active: simultaneous simulations (>= 1). */
+/* hbm_budget: device bytes the server may hold for simulation state: a
+ * simulation of n bodies holds 152 B per body (FP32 gravity arrays and FP64
+ * state) plus the batched gravity's partial sums for it (16 B per body per
+ * 4096-body superblock when n >= 32768, 16 B per body per j-piece when
+ * 16384 < n < 32768, else none).  The FP32 arena for budget / 152 bodies is
+ * reserved at creation.  max_active: simultaneous simulations (>= 1). */
 int khrr_server_create(khrr_ctx* ctx, size_t hbm_budget, int max_active, khrr_server** out);
 void khrr_server_destroy(khrr_server* srv);
+/* Round scheduling.  pairs == 0 (the default): every active simulation
+ * takes one step per round.  pairs > 0: the active simulations are taken
+ * in order of least remaining work (n^2 x steps left, ties by id) until
+ * the round holds at least `pairs` pair-interactions per step (always at
+ * least one); the others wait for a later round.  Results do not depend on
+ * the schedule. */
+int khrr_server_set_round_pairs(khrr_server* srv, double pairs);
 /* Copies the request's host arrays and queues it; *id identifies it.
  * Rejected (KHRR_ENOMEM) when it can never fit the budget. */
 int khrr_server_submit(khrr_server* srv, const khrr_request* req, long long* id);

@@ -186,6 +186,28 @@ BatchLauncher pick_batched_mode(bool a, bool p, bool c) {
   return c ? launch_batched<IPL, false, true, true> : launch_batched<IPL, false, true, false>;
 }
 
+template <int IPL>
+const void* pick_kernel_mode(bool a, bool p, bool c) {
+  if (a && p)
+    return c ? (const void*)batched_kernel<IPL, true, true, true>
+             : (const void*)batched_kernel<IPL, true, true, false>;
+  if (a)
+    return c ? (const void*)batched_kernel<IPL, true, false, true>
+             : (const void*)batched_kernel<IPL, true, false, false>;
+  return c ? (const void*)batched_kernel<IPL, false, true, true>
+           : (const void*)batched_kernel<IPL, false, true, false>;
+}
+
+const void* pick_kernel(int ipl, bool a, bool p, bool c) {
+  switch (ipl) {
+    case 1: return pick_kernel_mode<1>(a, p, c);
+    case 2: return pick_kernel_mode<2>(a, p, c);
+    case 4: return pick_kernel_mode<4>(a, p, c);
+    case 8: return pick_kernel_mode<8>(a, p, c);
+    default: return nullptr;
+  }
+}
+
 BatchLauncher pick_batched(int ipl, bool a, bool p, bool c) {
   switch (ipl) {
     case 1: return pick_batched_mode<1>(a, p, c);
@@ -194,6 +216,13 @@ BatchLauncher pick_batched(int ipl, bool a, bool p, bool c) {
     case 8: return pick_batched_mode<8>(a, p, c);
     default: return nullptr;
   }
+}
+
+// Systems of at least this many bodies take the pair-symmetric path (eight
+// or more superblocks: the one-sided diagonal items are at most 1/8 of
+// the system's work).  Like the piece count, a rule of the system's size.
+inline bool on_sym_path(int n) {
+  return n >= 8 * khrr::kSymSuperblock && khrr::sym_eligible(n, 8);
 }
 
 // Device work list of one composition.
@@ -205,12 +234,35 @@ struct WorkList {
   int n_red = 0;
   int max_red_count = 0;
   long long part_units = 0;  // sum of k*count over cut systems
+  khrr::SymItem* sym_items = nullptr;
+  int n_sym_items = 0;
+  khrr::SymSys* sym_sys = nullptr;
+  int n_sym_sys = 0;
+  int max_sym_n = 0;
+  long long sym_floats = 0;
 };
 
-// Builds (or reuses) the device work list for this composition: items
-// longest system first, the j-tile ranges of every system, the cut
-// systems' reduction list, then the tile flags (recomputed on every call:
-// masses and softening may change).
+// Byte offsets of the tables in the device block (and the pinned copy).
+struct Layout {
+  size_t items, tiles, red, sym_items, sym_sys, flags, host_bytes, dev_bytes;
+  Layout(size_t n_items, size_t n_tiles, size_t n_red, size_t n_sym_items, size_t n_sym_sys) {
+    auto need = [](size_t c, size_t e) { return khrr::Carver::need(c > 0 ? c : 1, e); };
+    items = 0;
+    tiles = items + need(n_items, sizeof(Item));
+    red = tiles + need(n_tiles, sizeof(int2));
+    sym_items = red + need(n_red, sizeof(RedItem));
+    sym_sys = sym_items + need(n_sym_items, sizeof(khrr::SymItem));
+    flags = sym_sys + need(n_sym_sys, sizeof(khrr::SymSys));
+    host_bytes = flags;
+    dev_bytes = flags + need(n_tiles, sizeof(int));
+  }
+};
+
+// Builds (or reuses) the device work list for this composition: one-sided
+// items longest system first, the flag ranges of every system (j-tiles, or
+// superblocks on the pair-symmetric path), the cut systems' reduction
+// list, the pair-symmetric items and systems; then the flags (recomputed on
+// every call: masses and softening may change).
 int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* host_count,
                   const float* h, const float* m, int ipl, int j_tile, WorkList* wl) {
   std::vector<int> key(2 * (size_t)n_sys + 1);
@@ -221,12 +273,26 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
   key[2 * (size_t)n_sys] = j_tile;
   const bool same = ctx->batch_ipl == ipl && ctx->batch_off == key;
   if (!same) {
+    constexpr int SB = khrr::kSymSuperblock;
     std::vector<int> flag_base(n_sys);
     std::vector<int2> tiles;
+    std::vector<khrr::SymItem> sym_items;
+    std::vector<khrr::SymSys> sym_sys;
+    long long sym_floats = 0;
+    int max_sym = 0;
     for (int s = 0; s < n_sys; ++s) {
       flag_base[s] = (int)tiles.size();
-      const int b = host_begin[s], e = host_begin[s] + host_count[s];
-      for (int jt = b; jt < e; jt += j_tile) tiles.push_back(make_int2(jt, std::min(e, jt + j_tile)));
+      const int b = host_begin[s], n = host_count[s], e = b + n;
+      const int step = on_sym_path(n) ? SB : j_tile;
+      for (int jt = b; jt < e; jt += step) tiles.push_back(make_int2(jt, std::min(e, jt + step)));
+      if (!on_sym_path(n)) continue;
+      const int n_sb = (n + SB - 1) / SB;
+      for (int A = 0; A < n_sb; ++A)
+        for (int B = A; B < n_sb; ++B)
+          sym_items.push_back(khrr::SymItem{b, n, A, B, flag_base[s], 0, sym_floats});
+      sym_sys.push_back(khrr::SymSys{b, n, n_sb, 0, sym_floats});
+      sym_floats += 4ll * n_sb * n;
+      max_sym = std::max(max_sym, n);
     }
     std::vector<int> order(n_sys);
     std::iota(order.begin(), order.end(), 0);
@@ -238,7 +304,7 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
     int max_red = 0;
     for (int s : order) {
       const int n = host_count[s], b = host_begin[s];
-      if (n == 0) continue;
+      if (n == 0 || on_sym_path(n)) continue;
       const int groups = (int)khrr::ceil_div(n, 64ll * ipl);
       // pieces as khrr_accel's j_split: chunk rounded up to whole tiles
       long long chunk = ((long long)n + pieces_of(n) - 1) / pieces_of(n);
@@ -257,33 +323,36 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
         max_red = std::max(max_red, n);
       }
     }
-    if (unit > INT32_MAX) return khrr::fail(ctx, KHRR_EINVAL, "batch too large for partial sums");
-    const size_t ib = khrr::Carver::need(list.size() > 0 ? list.size() : 1, sizeof(Item));
-    const size_t tb = khrr::Carver::need(tiles.size() > 0 ? tiles.size() : 1, sizeof(int2));
-    const size_t rb = khrr::Carver::need(red.size() > 0 ? red.size() : 1, sizeof(RedItem));
-    const size_t fb = khrr::Carver::need(tiles.size() > 0 ? tiles.size() : 1, sizeof(int));
-    const size_t host_bytes = ib + tb + rb;
+    if (unit > INT32_MAX || list.size() > INT32_MAX || sym_items.size() > INT32_MAX ||
+        tiles.size() > INT32_MAX)
+      return khrr::fail(ctx, KHRR_EINVAL, "batch too large");
+    const Layout L(list.size(), tiles.size(), red.size(), sym_items.size(), sym_sys.size());
     KHRR_HIP(ctx, hipStreamSynchronize(ctx->stream));  // old list/pinned copy may be in flight
-    if (ib + tb + rb + fb > ctx->batch_dev_bytes) {
+    if (L.dev_bytes > ctx->batch_dev_bytes) {
       if (ctx->batch_dev) KHRR_HIP(ctx, hipFree(ctx->batch_dev));
       ctx->batch_dev = nullptr;
       ctx->batch_dev_bytes = 0;
-      KHRR_HIP(ctx, hipMalloc(&ctx->batch_dev, ib + tb + rb + fb));
-      ctx->batch_dev_bytes = ib + tb + rb + fb;
+      KHRR_HIP(ctx, hipMalloc(&ctx->batch_dev, L.dev_bytes));
+      ctx->batch_dev_bytes = L.dev_bytes;
     }
-    if (host_bytes > ctx->batch_pinned_bytes) {
+    if (L.host_bytes > ctx->batch_pinned_bytes) {
       if (ctx->batch_pinned) KHRR_HIP(ctx, hipHostFree(ctx->batch_pinned));
       ctx->batch_pinned = nullptr;
       ctx->batch_pinned_bytes = 0;
-      KHRR_HIP(ctx, hipHostMalloc(&ctx->batch_pinned, host_bytes));
-      ctx->batch_pinned_bytes = host_bytes;
+      KHRR_HIP(ctx, hipHostMalloc(&ctx->batch_pinned, L.host_bytes));
+      ctx->batch_pinned_bytes = L.host_bytes;
     }
     char* pin = static_cast<char*>(ctx->batch_pinned);
-    char* dev = static_cast<char*>(ctx->batch_dev);
-    if (!list.empty()) std::memcpy(pin, list.data(), list.size() * sizeof(Item));
-    if (!tiles.empty()) std::memcpy(pin + ib, tiles.data(), tiles.size() * sizeof(int2));
-    if (!red.empty()) std::memcpy(pin + ib + tb, red.data(), red.size() * sizeof(RedItem));
-    KHRR_HIP(ctx, hipMemcpyAsync(dev, pin, host_bytes, hipMemcpyHostToDevice, ctx->stream));
+    auto put = [&](size_t off, const void* src, size_t bytes) {
+      if (bytes) std::memcpy(pin + off, src, bytes);
+    };
+    put(L.items, list.data(), list.size() * sizeof(Item));
+    put(L.tiles, tiles.data(), tiles.size() * sizeof(int2));
+    put(L.red, red.data(), red.size() * sizeof(RedItem));
+    put(L.sym_items, sym_items.data(), sym_items.size() * sizeof(khrr::SymItem));
+    put(L.sym_sys, sym_sys.data(), sym_sys.size() * sizeof(khrr::SymSys));
+    KHRR_HIP(ctx, hipMemcpyAsync(ctx->batch_dev, pin, L.host_bytes, hipMemcpyHostToDevice,
+                                 ctx->stream));
     ctx->batch_off = key;
     ctx->batch_ipl = ipl;
     ctx->batch_items = (int)list.size();
@@ -291,28 +360,64 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
     ctx->batch_red = (int)red.size();
     ctx->batch_red_max = max_red;
     ctx->batch_units = unit;
+    ctx->batch_sym_items = (int)sym_items.size();
+    ctx->batch_sym_sys = (int)sym_sys.size();
+    ctx->batch_sym_max = max_sym;
+    ctx->batch_sym_floats = sym_floats;
   }
-  const size_t ib = khrr::Carver::need(ctx->batch_items > 0 ? ctx->batch_items : 1, sizeof(Item));
-  const size_t tb = khrr::Carver::need(ctx->batch_tiles > 0 ? ctx->batch_tiles : 1, sizeof(int2));
-  const size_t rb = khrr::Carver::need(ctx->batch_red > 0 ? ctx->batch_red : 1, sizeof(RedItem));
+  const Layout L(ctx->batch_items, ctx->batch_tiles, ctx->batch_red, ctx->batch_sym_items,
+                 ctx->batch_sym_sys);
   char* dev = static_cast<char*>(ctx->batch_dev);
-  wl->items = reinterpret_cast<Item*>(dev);
-  const int2* tiles = reinterpret_cast<const int2*>(dev + ib);
-  wl->red = reinterpret_cast<RedItem*>(dev + ib + tb);
-  wl->tile_flags = reinterpret_cast<int*>(dev + ib + tb + rb);
+  wl->items = reinterpret_cast<Item*>(dev + L.items);
+  const int2* tiles = reinterpret_cast<const int2*>(dev + L.tiles);
+  wl->red = reinterpret_cast<RedItem*>(dev + L.red);
+  wl->sym_items = reinterpret_cast<khrr::SymItem*>(dev + L.sym_items);
+  wl->sym_sys = reinterpret_cast<khrr::SymSys*>(dev + L.sym_sys);
+  wl->tile_flags = reinterpret_cast<int*>(dev + L.flags);
   wl->n_items = ctx->batch_items;
   wl->n_red = ctx->batch_red;
   wl->max_red_count = ctx->batch_red_max;
   wl->part_units = ctx->batch_units;
+  wl->n_sym_items = ctx->batch_sym_items;
+  wl->n_sym_sys = ctx->batch_sym_sys;
+  wl->max_sym_n = ctx->batch_sym_max;
+  wl->sym_floats = ctx->batch_sym_floats;
   if (ctx->batch_tiles > 0)
     hipLaunchKernelGGL(tile_flags_kernel, dim3(khrr::ceil_div(ctx->batch_tiles, 4)), dim3(256), 0,
                        ctx->stream, ctx->batch_tiles, tiles, h, m, wl->tile_flags);
   return khrr::check_launch(ctx, "tile_flags_kernel");
 }
 
+// Resident blocks per CU of the persistent kernel for this shape (cached).
+int blocks_per_cu(int ipl, bool a, bool p, bool c, int wpb) {
+  static int cache[4][3][2][5] = {};
+  const int li = ipl == 1 ? 0 : ipl == 2 ? 1 : ipl == 4 ? 2 : 3;
+  const int mi = a && p ? 0 : a ? 1 : 2;
+  int& v = cache[li][mi][c][wpb];
+  if (v == 0) {
+    int nb = 0;
+    const void* f = pick_kernel(ipl, a, p, c);
+    if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * wpb, 0) != hipSuccess ||
+        nb < 1)
+      nb = std::max(1, 8 / wpb);
+    v = nb;
+  }
+  return v;
+}
+
 }  // namespace
 
 namespace khrr {
+
+size_t batched_part_bytes(int n, bool compensated) {
+  if (n <= 0) return 0;
+  if (on_sym_path(n)) {
+    const size_t n_sb = (size_t)(n + kSymSuperblock - 1) / kSymSuperblock;
+    return n_sb * 4 * (size_t)n * sizeof(float);
+  }
+  const int k = pieces_of(n);  // the piece count before tile rounding bounds it
+  return k > 1 ? (size_t)k * (compensated ? 8 : 4) * (size_t)n * sizeof(float) : 0;
+}
 
 int batched_segments(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* host_count,
                      int ld, const float* pos, const float* h, const float* m, float* acc,
@@ -326,21 +431,34 @@ int batched_segments(khrr_ctx* ctx, int n_sys, const int* host_begin, const int*
   BatchLauncher fn = pick_batched(t.i_per_lane, acc != nullptr, phi != nullptr, comp);
   if (!fn) return fail(ctx, KHRR_EINVAL, "unsupported i_per_lane %d", t.i_per_lane);
   float* part = nullptr;
-  if (wl.n_red > 0) {
+  float* sym_part = nullptr;
+  if (wl.n_red > 0 || wl.n_sym_items > 0) {
+    const size_t pb = Carver::need((size_t)(comp ? 8 : 4) * wl.part_units, sizeof(float));
+    const size_t sb = Carver::need((size_t)wl.sym_floats, sizeof(float));
     void* base = nullptr;
-    KHRR_TRY(scratch(ctx, Carver::need((size_t)(comp ? 8 : 4) * wl.part_units, sizeof(float)),
-                     &base));
-    part = static_cast<float*>(base);
+    KHRR_TRY(scratch(ctx, pb + sb, &base));
+    Carver cv(base);
+    part = cv.take<float>((size_t)(comp ? 8 : 4) * wl.part_units);
+    sym_part = cv.take<float>((size_t)wl.sym_floats);
   }
-  // Persistent grid: at most 8 waves per CU in flight, never more waves than items.
-  const long long waves = std::min<long long>(wl.n_items, 8ll * ctx->cu_count);
-  const int wpb = t.waves_per_block;
   const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
-  int* counter = ctx->dev_words;
   Timed tm(ctx, KHRR_K_GRAVITY);
+  // pair-symmetric systems first (one workgroup per (A, B) item) ...
+  KHRR_TRY(sym_items_launch(ctx, wl.n_sym_items, wl.sym_items, wl.n_sym_sys, wl.sym_sys,
+                            wl.max_sym_n, ld, pos, h, m, wl.tile_flags, sym_part, acc, phi,
+                            flags));
+  if (wl.n_items == 0) return 0;
+  // ... then the persistent one-sided grid: as many blocks as stay resident,
+  // never more waves than items
+  const int wpb = t.waves_per_block;
+  const long long resident =
+      (long long)blocks_per_cu(t.i_per_lane, acc != nullptr, phi != nullptr, comp, wpb) *
+      ctx->cu_count;
+  const long long blocks = std::min<long long>(resident, ceil_div(wl.n_items, wpb));
+  int* counter = ctx->dev_words;
   hipLaunchKernelGGL(set_word, dim3(1), dim3(1), 0, ctx->stream, counter, 0);
-  fn(dim3(ceil_div(waves, wpb)), dim3(64 * wpb), ctx->stream, wl.n_items, wl.items, wl.tile_flags,
-     ld, pos, h, m, t.j_tile, counter, acc, phi, part, accumulate);
+  fn(dim3((unsigned)blocks), dim3(64 * wpb), ctx->stream, wl.n_items, wl.items, wl.tile_flags, ld,
+     pos, h, m, t.j_tile, counter, acc, phi, part, accumulate);
   if (wl.n_red > 0) {
     const dim3 grid(ceil_div(wl.max_red_count, 256), wl.n_red);
     if (comp)

@@ -46,16 +46,14 @@ __device__ __forceinline__ void dev_fex(const ExtPot& e, double px, double py, d
   }
 }
 
+// One body of the predictor (istep 0) or corrector (istep 1).  N is the
+// plane stride of the FP64 arrays, L that of the FP32 arrays.
 template <bool EXT>
-__global__ __launch_bounds__(256) void integrate_kernel(
-    int n, int istep, double dt, ExtPot ext, const float* __restrict__ acc,
-    double* __restrict__ pos_t, double* __restrict__ vel_t, const double* __restrict__ pos,
-    const double* __restrict__ vel, float* __restrict__ acc0, float* __restrict__ pos32,
-    int ld32) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const size_t N = (size_t)n;
-  const size_t L = (size_t)ld32;  // plane stride of the FP32 arrays
+__device__ __forceinline__ void integrate_body(
+    int i, size_t N, size_t L, int istep, double dt, const ExtPot& ext,
+    const float* __restrict__ acc, double* __restrict__ pos_t, double* __restrict__ vel_t,
+    const double* __restrict__ pos, const double* __restrict__ vel, float* __restrict__ acc0,
+    float* __restrict__ pos32) {
   const double rx = pos[i], ry = pos[N + i], rz = pos[2 * N + i];
   const double vx = vel[i], vy = vel[N + i], vz = vel[2 * N + i];
   double ex = 0.0, ey = 0.0, ez = 0.0;
@@ -106,6 +104,51 @@ __global__ __launch_bounds__(256) void integrate_kernel(
   }
 }
 
+template <bool EXT>
+__global__ __launch_bounds__(256) void integrate_kernel(
+    int n, int istep, double dt, ExtPot ext, const float* __restrict__ acc,
+    double* __restrict__ pos_t, double* __restrict__ vel_t, const double* __restrict__ pos,
+    const double* __restrict__ vel, float* __restrict__ acc0, float* __restrict__ pos32,
+    int ld32) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  integrate_body<EXT>(i, (size_t)n, (size_t)ld32, istep, dt, ext, acc, pos_t, vel_t, pos, vel,
+                      acc0, pos32);
+}
+
+// Many systems in one launch: block b belongs to the last segment whose
+// first_block <= b.  Each segment's FP64 state is one allocation of planes
+// pos, vel, pos_t, vel_t (3n each); the FP32 arrays are shared planes of
+// stride ld32 with the segment at `begin`.  istep 0 reads pos/vel and
+// writes pos_t/vel_t; istep 1 reads pos_t/vel_t and writes pos/vel, exactly
+// as the single-system predictor and corrector calls.
+__global__ __launch_bounds__(256) void integrate_segments_kernel(
+    int n_seg, const khrr::IntSeg* __restrict__ segs, int istep, const float* __restrict__ acc,
+    float* __restrict__ acc0, float* __restrict__ pos32, int ld32) {
+  int lo = 0, hi = n_seg - 1;
+  const int b = blockIdx.x;
+  while (lo < hi) {  // uniform per block
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].first_block <= b) lo = mid; else hi = mid - 1;
+  }
+  const khrr::IntSeg sg = segs[lo];
+  const int i = (b - sg.first_block) * blockDim.x + threadIdx.x;
+  if (i >= sg.n) return;
+  const size_t N = (size_t)sg.n;
+  double* P = sg.state;
+  double* V = sg.state + 3 * N;
+  double* PT = sg.state + 6 * N;
+  double* VT = sg.state + 9 * N;
+  const size_t off = (size_t)sg.begin;
+  const ExtPot none{};
+  if (istep == 0)
+    integrate_body<false>(i, N, (size_t)ld32, 0, sg.dt, none, acc + off, PT, VT, P, V, nullptr,
+                          pos32 ? pos32 + off : nullptr);
+  else
+    integrate_body<false>(i, N, (size_t)ld32, 1, sg.dt, none, acc + off, P, V, PT, VT,
+                          acc0 ? acc0 + off : nullptr, pos32 ? pos32 + off : nullptr);
+}
+
 }  // namespace
 
 namespace khrr {
@@ -135,6 +178,15 @@ int integrate_launch(khrr_ctx* ctx, int n, int istep, double dt, const khrr_extp
     hipLaunchKernelGGL(integrate_kernel<false>, grid, block, 0, ctx->stream, n, istep, dt, e, acc,
                        pos_t, vel_t, pos, vel, acc0, pos32, ld32);
   return check_launch(ctx, "integrate_kernel");
+}
+
+int integrate_segments(khrr_ctx* ctx, int n_seg, const IntSeg* segs, int total_blocks, int istep,
+                       const float* acc, float* acc0, float* pos32, int ld32) {
+  if (n_seg == 0 || total_blocks == 0) return 0;
+  Timed tm(ctx, KHRR_K_INTEGRATE);
+  hipLaunchKernelGGL(integrate_segments_kernel, dim3(total_blocks), dim3(256), 0, ctx->stream,
+                     n_seg, segs, istep, acc, acc0, pos32, ld32);
+  return check_launch(ctx, "integrate_segments_kernel");
 }
 
 }  // namespace khrr

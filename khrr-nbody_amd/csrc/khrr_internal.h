@@ -16,6 +16,28 @@
 namespace khrr {
 
 constexpr int kWave = 64;  // CDNA wavefront width
+constexpr int kSymSuperblock = 4096;  // bodies per superblock of the pair-symmetric form
+
+// Pair-symmetric batched form: one (A, B) superblock item of the system
+// whose bodies start at `base`; its superblock flags start at
+// flags[flag_base] and its partial slots at part[part] (n_sb*4*n floats).
+struct SymItem {
+  int base, n, A, B, flag_base, pad;
+  long long part;
+};
+// One system of a many-system integrator launch: FP64 state planes pos,
+// vel, pos_t, vel_t (3n each) from `state`, FP32 planes at offset `begin`,
+// blocks [first_block, first_block + ceil(n/256)).
+struct IntSeg {
+  double* state;
+  double dt;
+  int n, begin, first_block, pad;
+};
+// A system of the pair-symmetric batched form, for the slot reduction.
+struct SymSys {
+  int base, n, n_sb, pad;
+  long long part;
+};
 
 struct TimerPair {
   hipEvent_t start;
@@ -58,6 +80,10 @@ struct khrr_ctx {
   int batch_red = 0;          // systems whose j-range is cut into pieces
   int batch_red_max = 0;      // largest such system
   long long batch_units = 0;  // partial-sum rows of all cut systems
+  int batch_sym_items = 0;    // pair-symmetric (A, B) items
+  int batch_sym_sys = 0;      // systems on the pair-symmetric path
+  int batch_sym_max = 0;      // largest such system
+  long long batch_sym_floats = 0;  // their partial slots
   void* batch_dev = nullptr;  // items, j-tile ranges, cut systems, tile flags
   size_t batch_dev_bytes = 0;
   void* batch_pinned = nullptr;
@@ -124,16 +150,29 @@ int accel_launch(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, i
 int integrate_launch(khrr_ctx* ctx, int n, int istep, double dt, const khrr_extpot* ext,
                      const float* acc, double* pos_t, double* vel_t, const double* pos,
                      const double* vel, float* acc0, float* pos32, int ld32);
+// Predictor (istep 0) or corrector (istep 1) of many systems in one launch
+// over a device segment table (no external potential).
+int integrate_segments(khrr_ctx* ctx, int n_seg, const IntSeg* segs, int total_blocks, int istep,
+                       const float* acc, float* acc0, float* pos32, int ld32);
 // Batched form over arbitrary segments [begin, begin+count) of planes with
 // stride ld (khrr_batch.hip); host arrays describe the segments.
 int batched_segments(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* host_count,
                      int ld, const float* pos, const float* h, const float* m, float* acc,
                      float* phi, unsigned flags, const khrr_tune& t);
+// Scratch bytes the batched form needs for the partial sums of one system
+// of n bodies (0 when it writes its outputs directly).
+size_t batched_part_bytes(int n, bool compensated);
 // Pair-symmetric form (khrr_sym.hip) for one system (i-set == j-set).
 bool sym_eligible(int n, int ipl);
 bool sym_worthwhile(const khrr_ctx* ctx, int n);
 int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const float* m, float* acc,
                float* phi, unsigned flags, int ipl);
+// Batched pair-symmetric launch (i_per_lane 8) over device item and system
+// tables, then the per-system slot reduction; max_n bounds every system.
+int sym_items_launch(khrr_ctx* ctx, int n_items, const SymItem* items, int n_sys,
+                     const SymSys* sys, int max_n, int ld, const float* pos, const float* h,
+                     const float* m, const int* sb_flags, float* part, float* acc, float* phi,
+                     unsigned flags);
 
 }  // namespace khrr
 

@@ -11,8 +11,9 @@
 //              as the reference (subtract, IEEE divide, truncate with GPU
 //              cvt saturation, clamp; z clamped with size[1]; y stride sz).
 //  2. sort:    stable LSD radix sort of (cell, particle index), 8-bit
-//              digits; in-block ranks come from 64-lane ballot matching, so
-//              equal cells keep index order.
+//              digits; per pass a tile histogram, one scan block per digit
+//              row, and a scatter whose in-block ranks come from 64-lane
+//              ballot matching, so equal cells keep index order.
 //  3. cells:   one 512-thread block per 512-cell block of the reference.
 //              Two binary searches on the sorted cells give the number of
 //              particles before the block (= the reference's block-granular
@@ -21,6 +22,7 @@
 //              non-empty cell walks its sorted segment in index order to
 //              write id_in_cell and the FP64 mass sum.
 // Every cell is written exactly once, so no output needs clearing first.
+#include <algorithm>
 #include <climits>
 
 #include "khrr_internal.h"
@@ -82,28 +84,44 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist(int n, const int* __r
   hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// In-place exclusive scan of len ints by one 1024-thread block.
-__global__ __launch_bounds__(1024) void scan_exclusive(int len, int* __restrict__ data) {
-  __shared__ int sums[1024];
-  const int per = (len + 1023) / 1024;
-  const int t0 = threadIdx.x * per;
-  const int t1 = min(len, t0 + per);
-  int s = 0;
-  for (int k = t0; k < t1; ++k) s += data[k];
-  sums[threadIdx.x] = s;
+// Exclusive scan of 256 values, one per thread of a 256-thread block;
+// *total receives the sum.  wsum: 4 ints of LDS.
+__device__ __forceinline__ int block_scan256(int v, int* wsum, int* total) {
+  const int lane = threadIdx.x & (khrr::kWave - 1);
+  const int w = threadIdx.x / khrr::kWave;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < khrr::kWave; o <<= 1) {
+    const int y = __shfl_up(x, o, khrr::kWave);
+    if (lane >= o) x += y;
+  }
+  if (lane == khrr::kWave - 1) wsum[w] = x;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = threadIdx.x >= (unsigned)off ? sums[threadIdx.x - off] : 0;
-    __syncthreads();
-    sums[threadIdx.x] += v;
-    __syncthreads();
+  int before = 0;
+  for (int k = 0; k < w; ++k) before += wsum[k];
+  *total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  __syncthreads();  // wsum is free again for the next call
+  return before + x - v;
+}
+
+// hist is digit-major, hist[d * ntiles + tile].  Block d scans row d in
+// place (exclusive, tile order) and writes the row sum to totals[d]; the
+// scatter adds the exclusive prefix of totals, so the whole array is
+// scanned in digit-then-tile order with 256 blocks instead of one.
+__global__ __launch_bounds__(256) void scan_rows(int ntiles, int* __restrict__ hist,
+                                                 int* __restrict__ totals) {
+  __shared__ int wsum[4];
+  int* row = hist + (size_t)blockIdx.x * ntiles;
+  int carry = 0;
+  for (int base = 0; base < ntiles; base += 256) {
+    const int i = base + (int)threadIdx.x;
+    const int v = i < ntiles ? row[i] : 0;
+    int total = 0;
+    const int ex = block_scan256(v, wsum, &total);
+    if (i < ntiles) row[i] = carry + ex;
+    carry += total;
   }
-  int run = threadIdx.x > 0 ? sums[threadIdx.x - 1] : 0;
-  for (int k = t0; k < t1; ++k) {
-    const int v = data[k];
-    data[k] = run;
-    run += v;
-  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
 // Stable scatter of one digit pass.  Items are visited in index order
@@ -112,16 +130,19 @@ __global__ __launch_bounds__(1024) void scan_exclusive(int len, int* __restrict_
 // per-digit position across waves and rounds.
 __global__ __launch_bounds__(kSortThreads) void radix_scatter(
     int n, const int* __restrict__ keys_in, const int* __restrict__ vals_in, int shift,
-    int ntiles, const int* __restrict__ offs, int* __restrict__ keys_out,
-    int* __restrict__ vals_out) {
+    int ntiles, const int* __restrict__ offs, const int* __restrict__ totals,
+    int* __restrict__ keys_out, int* __restrict__ vals_out) {
   constexpr int kWaves = kSortThreads / khrr::kWave;
   __shared__ int run[kRadix];
   __shared__ int wcnt[kWaves][kRadix];
   __shared__ int wpre[kWaves][kRadix];
+  __shared__ int wsum[4];
   const int tid = threadIdx.x;
   const int lane = tid & (khrr::kWave - 1);
   const int wave = tid / khrr::kWave;
-  run[tid] = offs[tid * ntiles + blockIdx.x];
+  int all = 0;
+  const int digit_base = block_scan256(totals[tid], wsum, &all);
+  run[tid] = digit_base + offs[(size_t)tid * ntiles + blockIdx.x];
   for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
   __syncthreads();
   const unsigned long long lt = (1ull << lane) - 1ull;
@@ -181,58 +202,75 @@ __global__ __launch_bounds__(256) void block_bounds(int n, int nb, const int* __
   if (b <= nb) bounds[b] = lower_bound_dev(skeys, n, (long long)b * kCellBlock);
 }
 
+// One 512-cell block of the reference per loop trip; a resident grid walks
+// the blocks with a stride and loads the next block's bounds while it
+// writes the current one (most blocks of a fine grid are empty and are just
+// four coalesced stores per cell).
 __global__ __launch_bounds__(kCellBlock) void bin_cells(
-    int n, long long G, const int* __restrict__ skeys, const int* __restrict__ svals,
+    int n, long long G, int cblocks, const int* __restrict__ skeys, const int* __restrict__ svals,
     const int* __restrict__ bounds, const double* __restrict__ mass, int* __restrict__ count,
     int* __restrict__ offset, int* __restrict__ start_ref, int* __restrict__ block_incl,
     int* __restrict__ id_in_cell, double* __restrict__ cell_mass) {
   __shared__ int cnt[kCellBlock];
   __shared__ int wsum[kCellBlock / khrr::kWave];
   const int tid = threadIdx.x;
-  const long long c0 = (long long)blockIdx.x * kCellBlock;
-  const long long c = c0 + tid;
-  const int lo = bounds[blockIdx.x], hi = bounds[blockIdx.x + 1];
-  if (tid == 0 && block_incl) block_incl[blockIdx.x] = hi;
-  if (lo == hi) {
-    // empty 512-cell block (most of a fine grid): plain coalesced stores
-    if (c < G) {
-      count[c] = 0;
-      if (offset) offset[c] = lo;
-      if (start_ref) start_ref[c] = lo;
-      if (cell_mass) cell_mass[c] = 0.0;
-    }
-    return;
-  }
-  cnt[tid] = 0;
-  __syncthreads();
-  for (int p = lo + tid; p < hi; p += kCellBlock) atomicAdd(&cnt[skeys[p] - (int)c0], 1);
-  __syncthreads();
-  // Exclusive scan of cnt: wave-inclusive scans, then wave totals.
   const int lane = tid & (khrr::kWave - 1);
   const int wave = tid / khrr::kWave;
-  const int mine = cnt[tid];
-  int incl = mine;
-#pragma unroll
-  for (int off = 1; off < khrr::kWave; off <<= 1) {
-    const int v = __shfl_up(incl, off, khrr::kWave);
-    if (lane >= off) incl += v;
-  }
-  if (lane == khrr::kWave - 1) wsum[wave] = incl;
-  __syncthreads();
-  int before = 0;
-  for (int w = 0; w < wave; ++w) before += wsum[w];
-  const int excl = lo + before + incl - mine;
-  if (c < G) {
-    count[c] = mine;
-    if (offset) offset[c] = excl;
-    if (start_ref) start_ref[c] = lo;
-    double m = 0.0;
-    for (int k = 0; k < mine; ++k) {
-      const int v = svals[excl + k];
-      if (id_in_cell) id_in_cell[v] = k;
-      if (cell_mass) m += mass[v];
+  int b = blockIdx.x;
+  if (b >= cblocks) return;
+  int lo = bounds[b], hi = bounds[b + 1];
+  while (b < cblocks) {
+    const int nb = b + (int)gridDim.x;
+    int nlo = 0, nhi = 0;
+    if (nb < cblocks) {
+      nlo = bounds[nb];
+      nhi = bounds[nb + 1];
     }
-    if (cell_mass) cell_mass[c] = m;
+    const long long c0 = (long long)b * kCellBlock;
+    const long long c = c0 + tid;
+    if (tid == 0 && block_incl) block_incl[b] = hi;
+    if (lo == hi) {
+      // empty block (lo, hi are the same for every thread)
+      if (c < G) {
+        count[c] = 0;
+        if (offset) offset[c] = lo;
+        if (start_ref) start_ref[c] = lo;
+        if (cell_mass) cell_mass[c] = 0.0;
+      }
+    } else {
+      cnt[tid] = 0;
+      __syncthreads();
+      for (int p = lo + tid; p < hi; p += kCellBlock) atomicAdd(&cnt[skeys[p] - (int)c0], 1);
+      __syncthreads();
+      // Exclusive scan of cnt: wave-inclusive scans, then wave totals.
+      const int mine = cnt[tid];
+      int incl = mine;
+#pragma unroll
+      for (int off = 1; off < khrr::kWave; off <<= 1) {
+        const int v = __shfl_up(incl, off, khrr::kWave);
+        if (lane >= off) incl += v;
+      }
+      if (lane == khrr::kWave - 1) wsum[wave] = incl;
+      __syncthreads();
+      int before = 0;
+      for (int w = 0; w < wave; ++w) before += wsum[w];
+      const int excl = lo + before + incl - mine;
+      if (c < G) {
+        count[c] = mine;
+        if (offset) offset[c] = excl;
+        if (start_ref) start_ref[c] = lo;
+        double m = 0.0;
+        for (int k = 0; k < mine; ++k) {
+          const int v = svals[excl + k];
+          if (id_in_cell) id_in_cell[v] = k;
+          if (cell_mass) m += mass[v];
+        }
+        if (cell_mass) cell_mass[c] = m;
+      }
+    }
+    b = nb;
+    lo = nlo;
+    hi = nhi;
   }
 }
 
@@ -325,6 +363,7 @@ extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, cons
   const int cblocks = (int)ceil_div(G, kCellBlock);
   const size_t need = 5 * Carver::need((size_t)(n > 0 ? n : 1), sizeof(int)) +
                       Carver::need((size_t)kRadix * ntiles, sizeof(int)) +
+                      Carver::need((size_t)kRadix, sizeof(int)) +
                       Carver::need((size_t)cblocks + 1, sizeof(int));
   void* base = nullptr;
   KHRR_TRY(scratch(ctx, need, &base));
@@ -335,6 +374,7 @@ extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, cons
   int* kb = cv.take<int>(n > 0 ? n : 1);
   int* vb = cv.take<int>(n > 0 ? n : 1);
   int* hist = cv.take<int>((size_t)kRadix * ntiles);
+  int* totals = cv.take<int>((size_t)kRadix);
   int* bounds = cv.take<int>((size_t)cblocks + 1);
   if (cell_id) cells = cell_id;
 
@@ -357,10 +397,10 @@ extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, cons
       int* vout = (p & 1) ? vb : va;
       hipLaunchKernelGGL(radix_hist, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, kin,
                          p * kRadixBits, ntiles, hist);
-      hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(1024), 0, ctx->stream, kRadix * ntiles,
-                         hist);
+      hipLaunchKernelGGL(scan_rows, dim3(kRadix), dim3(256), 0, ctx->stream, ntiles, hist,
+                         totals);
       hipLaunchKernelGGL(radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, kin,
-                         vin, p * kRadixBits, ntiles, hist, kout, vout);
+                         vin, p * kRadixBits, ntiles, hist, totals, kout, vout);
       kin = kout;
       vin = vout;
     }
@@ -370,18 +410,19 @@ extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, cons
       // A single cell: the identity permutation is already sorted.
       hipLaunchKernelGGL(radix_hist, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, cells,
                          0, ntiles, hist);
-      hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(1024), 0, ctx->stream, kRadix * ntiles,
-                         hist);
+      hipLaunchKernelGGL(scan_rows, dim3(kRadix), dim3(256), 0, ctx->stream, ntiles, hist,
+                         totals);
       hipLaunchKernelGGL(radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n,
-                         cells, nullptr, 0, ntiles, hist, ka, va);
+                         cells, nullptr, 0, ntiles, hist, totals, ka, va);
       skeys = ka;
       svals = va;
     }
   }
   hipLaunchKernelGGL(block_bounds, dim3(ceil_div(cblocks + 1, 256)), dim3(256), 0, ctx->stream, n,
                      cblocks, skeys, bounds);
-  hipLaunchKernelGGL(bin_cells, dim3(cblocks), dim3(kCellBlock), 0, ctx->stream, n, G, skeys,
-                     svals, bounds, mass, count, offset, start_id_ref, block_incl, id_in_cell,
-                     cell_mass);
+  const int cgrid = (int)std::min<long long>(cblocks, 8ll * ctx->cu_count);
+  hipLaunchKernelGGL(bin_cells, dim3(cgrid), dim3(kCellBlock), 0, ctx->stream, n, G, cblocks,
+                     skeys, svals, bounds, mass, count, offset, start_id_ref, block_incl,
+                     id_in_cell, cell_mass);
   return check_launch(ctx, "p2mesh kernels");
 }

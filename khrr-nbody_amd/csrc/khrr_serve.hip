@@ -10,8 +10,10 @@
 //      when its footprint fits the budget, a contiguous arena range is free
 //      and an active slot is free; new systems get their initial force,
 //      potential and t = 0 diagnostics;
-//   2. one step for every active system: predictor (per system), ONE
-//      batched gravity launch over all active systems, corrector;
+//   2. one step for the round's systems: every active system, or (with a
+//      round_pairs target) those with the least remaining work until the
+//      round holds that many pair-interactions; ONE predictor launch, ONE
+//      batched gravity launch and ONE corrector launch over all of them;
 //   3. systems that reached their step count get their potential, final
 //      diagnostics and the p2mesh mass grid, and release their memory.
 // A system's arithmetic never mixes with another's (segment-local j loops,
@@ -31,6 +33,10 @@ namespace {
 constexpr size_t kArenaFloatsPerBody = 12;  // pos32 3, m32, h32, acc 3, acct 3, phi
 constexpr size_t kStateDoublesPerBody = 13; // pos 3, vel 3, post 3, velt 3, mass
 constexpr size_t kBytesPerBody = kArenaFloatsPerBody * 4 + kStateDoublesPerBody * 8;  // 152
+
+// Device bytes a simulation of n bodies holds while active: arena share,
+// FP64 state and the batched gravity's partial sums for it.
+size_t footprint(int n) { return kBytesPerBody * (size_t)n + khrr::batched_part_bytes(n, false); }
 
 enum class State { kPending, kActive, kFinished };
 
@@ -117,6 +123,11 @@ struct khrr_server {
   double* bin_mass = nullptr;
   size_t bin_cells = 0;
   khrr_tune tune{4, 4, 1, 256, 0};
+  // Round scheduling: least remaining work first until the round holds
+  // round_pairs pair-interactions per step (0: every active system).
+  double round_pairs = 0.0;
+  khrr::IntSeg* seg_dev = nullptr;  // integrator segment table (max_active)
+  khrr::IntSeg* seg_pin = nullptr;
 };
 
 namespace {
@@ -155,7 +166,7 @@ int upload(khrr_server* s, Sim* m) {
 int admit(khrr_server* s, std::vector<Sim*>& admitted) {
   while (!s->pending.empty() && (int)s->active.size() < s->max_active) {
     Sim* m = s->pending.front();
-    const size_t foot = kBytesPerBody * (size_t)m->n;
+    const size_t foot = footprint(m->n);
     if (s->in_use + foot > s->budget) break;  // first come, first served: wait
     const int b = s->ranges.take(m->n);
     if (b < 0) break;
@@ -219,7 +230,7 @@ int finish(khrr_server* s, const std::vector<Sim*>& done, long long* ids, int ma
     KHRR_HIP(ctx, hipFree(m->dev));
     m->dev = nullptr;
     s->ranges.give(m->begin, m->n);
-    s->in_use -= kBytesPerBody * (size_t)m->n;
+    s->in_use -= footprint(m->n);
     m->state = State::kFinished;
     if (*n_ids < max_ids && ids) ids[*n_ids] = m->id;
     ++*n_ids;
@@ -263,6 +274,14 @@ int khrr_server_create(khrr_ctx* ctx, size_t hbm_budget, int max_active, khrr_se
   p += 3 * cap;
   s->phi = p;
   s->ranges.init((int)cap);
+  if (hipMalloc(&s->seg_dev, sizeof(khrr::IntSeg) * max_active) != hipSuccess ||
+      hipHostMalloc(&s->seg_pin, sizeof(khrr::IntSeg) * max_active) != hipSuccess) {
+    (void)hipGetLastError();
+    if (s->seg_dev) (void)hipFree(s->seg_dev);
+    (void)hipFree(s->arena);
+    delete s;
+    return khrr::fail(ctx, KHRR_ENOMEM, "server tables for %d simulations", max_active);
+  }
   if (ctx->has_batch_tuned) s->tune = ctx->batch_tuned;
   s->tune.j_split = 1;
   s->tune.symmetric = 0;
@@ -277,9 +296,18 @@ void khrr_server_destroy(khrr_server* s) {
   for (auto& kv : s->sims)
     if (kv.second->dev) (void)hipFree(kv.second->dev);
   if (s->arena) (void)hipFree(s->arena);
+  if (s->seg_dev) (void)hipFree(s->seg_dev);
+  if (s->seg_pin) (void)hipHostFree(s->seg_pin);
   if (s->bin_count) (void)hipFree(s->bin_count);
   if (s->bin_mass) (void)hipFree(s->bin_mass);
   delete s;
+}
+
+int khrr_server_set_round_pairs(khrr_server* s, double pairs) {
+  if (!s) return khrr::fail(nullptr, KHRR_EINVAL, "null server");
+  if (!(pairs >= 0.0)) return khrr::fail(s->ctx, KHRR_EINVAL, "round_pairs must be >= 0");
+  s->round_pairs = pairs;
+  return 0;
 }
 
 int khrr_server_submit(khrr_server* s, const khrr_request* req, long long* id) {
@@ -293,7 +321,7 @@ int khrr_server_submit(khrr_server* s, const khrr_request* req, long long* id) {
   const long long G = (long long)g.size[0] * g.size[1] * g.size[2];
   if (g.size[0] < 1 || g.size[1] < 1 || g.size[2] < 1 || G > INT32_MAX)
     return khrr::fail(ctx, KHRR_EINVAL, "bad result grid");
-  if (req->n > s->cap || kBytesPerBody * (size_t)req->n > s->budget)
+  if (req->n > s->cap || footprint(req->n) > s->budget)
     return khrr::fail(ctx, KHRR_ENOMEM, "request of %d bodies can never fit the budget", req->n);
   auto m = std::make_unique<Sim>();
   const size_t n = (size_t)req->n;
@@ -329,17 +357,37 @@ int khrr_server_step(khrr_server* s, long long* done_ids, int max_done, int* n_d
   KHRR_TRY(admit(s, admitted));
   std::vector<Sim*> stepping, done;
   for (Sim* m : s->active) (m->done_steps < m->steps ? stepping : done).push_back(m);
+  if (s->round_pairs > 0.0 && stepping.size() > 1) {
+    auto left = [](const Sim* m) { return (double)m->n * m->n * (m->steps - m->done_steps); };
+    std::stable_sort(stepping.begin(), stepping.end(), [&](const Sim* a, const Sim* b) {
+      const double la = left(a), lb = left(b);
+      return la != lb ? la < lb : a->id < b->id;
+    });
+    double pairs = 0.0;
+    size_t k = 0;
+    while (k < stepping.size() && (k == 0 || pairs < s->round_pairs)) {
+      pairs += (double)stepping[k]->n * stepping[k]->n;
+      ++k;
+    }
+    stepping.resize(k);
+  }
   if (!stepping.empty()) {
     // predictor -> one batched gravity launch -> corrector (nbody.cu:1525-1582)
-    for (Sim* m : stepping)
-      KHRR_TRY(khrr::integrate_launch(ctx, m->n, 0, m->dt, nullptr, s->acc + m->begin, m->d_post(),
-                                      m->d_velt(), m->d_pos(), m->d_vel(), nullptr,
-                                      s->pos32 + m->begin, s->cap));
+    int blocks = 0;
+    for (size_t k = 0; k < stepping.size(); ++k) {
+      Sim* m = stepping[k];
+      s->seg_pin[k] = khrr::IntSeg{m->dev, m->dt, m->n, m->begin, blocks, 0};
+      blocks += (int)khrr::ceil_div(m->n, 256);
+    }
+    const int n_seg = (int)stepping.size();
+    KHRR_HIP(ctx, hipMemcpyAsync(s->seg_dev, s->seg_pin, n_seg * sizeof(khrr::IntSeg),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    KHRR_TRY(khrr::integrate_segments(ctx, n_seg, s->seg_dev, blocks, 0, s->acc, nullptr,
+                                      s->pos32, s->cap));
     KHRR_TRY(gravity_over(s, stepping, s->acct, nullptr));
+    KHRR_TRY(khrr::integrate_segments(ctx, n_seg, s->seg_dev, blocks, 1, s->acct, s->acc, nullptr,
+                                      s->cap));
     for (Sim* m : stepping) {
-      KHRR_TRY(khrr::integrate_launch(ctx, m->n, 1, m->dt, nullptr, s->acct + m->begin, m->d_pos(),
-                                      m->d_vel(), m->d_post(), m->d_velt(), s->acc + m->begin,
-                                      nullptr, s->cap));
       ++m->done_steps;
       s->pairs += m->n * (double)m->n;
       if (m->done_steps >= m->steps) done.push_back(m);

@@ -32,7 +32,7 @@ using khrr::fold;
 using khrr::kUniformH;
 using khrr::kUniformM;
 
-constexpr int kSB = 4096;  // bodies per superblock
+constexpr int kSB = khrr::kSymSuperblock;
 
 __device__ __forceinline__ float ror1(float v) {
   // DPP wave_ror:1 (dpp_ctrl 0x13C): every lane receives its neighbour's
@@ -98,31 +98,21 @@ __device__ __forceinline__ void chunk64(float& xj, float& yj, float& zj, float& 
   }
 }
 
-// part layout: part[(X*4 + c)*n + i], c = ax, ay, az (acceleration from the
-// bodies of superblock X) and 3 = sum_j m_j / r (positive).
+// One (A, B) item of one system of n bodies whose planes start at xs/ys/zs,
+// m, h (any stride between planes); sb_flags[X] are the system's superblock
+// flags and part its slot block, laid out part[(X*4 + c)*n + i] with
+// c = ax, ay, az (acceleration from the bodies of superblock X) and
+// 3 = sum_j m_j / r (positive).
 template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
-__global__ __launch_bounds__(kSB / IPL) void sym_kernel(int n, int n_sb,
-                                                        const float* __restrict__ pos,
-                                                        const float* __restrict__ m,
-                                                        const float* __restrict__ h,
-                                                        const int* __restrict__ sb_flags,
-                                                        float* __restrict__ part) {
+__device__ __forceinline__ void sym_block(
+    int n, int A, int B, const float* __restrict__ xs, const float* __restrict__ ys,
+    const float* __restrict__ zs, const float* __restrict__ m, const float* __restrict__ h,
+    const int* __restrict__ sb_flags, float* __restrict__ part,
+    float (&red)[kSB / (khrr::kWave * IPL)][4][khrr::kWave]) {
   constexpr int W = kSB / (khrr::kWave * IPL);
-  __shared__ float red[W][4][khrr::kWave];
-  // item -> (A, B), row-major over the upper triangle
-  int t = blockIdx.x, A = 0, row = n_sb;
-  while (t >= row) {
-    t -= row;
-    ++A;
-    --row;
-  }
-  const int B = A + t;
   const int lane = threadIdx.x & (khrr::kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / khrr::kWave);
   const size_t N = (size_t)n;
-  const float* xs = pos;
-  const float* ys = pos + N;
-  const float* zs = pos + 2 * N;
   const int flags = sb_flags[A] & sb_flags[B];
 
   float xi[IPL], yi[IPL], zi[IPL], hi[IPL], mi[IPL];
@@ -216,6 +206,41 @@ __global__ __launch_bounds__(kSB / IPL) void sym_kernel(int n, int n_sb,
   }
 }
 
+template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
+__global__ __launch_bounds__(kSB / IPL) void sym_kernel(int n, int n_sb,
+                                                        const float* __restrict__ pos,
+                                                        const float* __restrict__ m,
+                                                        const float* __restrict__ h,
+                                                        const int* __restrict__ sb_flags,
+                                                        float* __restrict__ part) {
+  __shared__ float red[kSB / (khrr::kWave * IPL)][4][khrr::kWave];
+  // item -> (A, B), row-major over the upper triangle
+  int t = blockIdx.x, A = 0, row = n_sb;
+  while (t >= row) {
+    t -= row;
+    ++A;
+    --row;
+  }
+  const size_t N = (size_t)n;
+  sym_block<IPL, DO_ACC, DO_PHI, COMP>(n, A, A + t, pos, pos + N, pos + 2 * N, m, h, sb_flags,
+                                       part, red);
+}
+
+// Batched form: workgroup b takes items[b], an (A, B) item of one system of
+// a segment table (planes of stride ld).
+template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
+__global__ __launch_bounds__(kSB / IPL) void sym_items_kernel(
+    const khrr::SymItem* __restrict__ items, int ld, const float* __restrict__ pos,
+    const float* __restrict__ m, const float* __restrict__ h, const int* __restrict__ flags,
+    float* __restrict__ part) {
+  __shared__ float red[kSB / (khrr::kWave * IPL)][4][khrr::kWave];
+  const khrr::SymItem it = items[blockIdx.x];
+  const size_t b = (size_t)it.base;
+  sym_block<IPL, DO_ACC, DO_PHI, COMP>(it.n, it.A, it.B, pos + b, pos + ld + b,
+                                       pos + 2 * (size_t)ld + b, m + b, h + b,
+                                       flags + it.flag_base, part + it.part, red);
+}
+
 // acc[i] (+)= sum_X part[X][0..2][i];  phi[i] (-)= sum_X part[X][3][i]
 template <bool COMP>
 __global__ __launch_bounds__(256) void sym_reduce(int n, int n_sb, int do_acc, int do_phi,
@@ -235,6 +260,33 @@ __global__ __launch_bounds__(256) void sym_reduce(int n, int n_sb, int do_acc, i
       *a = accumulate ? *a + v : v;
     } else {
       phi[i] = accumulate ? phi[i] - v : -v;
+    }
+  }
+}
+
+// Batched form of sym_reduce: blockIdx.y selects the system.
+template <bool COMP>
+__global__ __launch_bounds__(256) void sym_reduce_sys(const khrr::SymSys* __restrict__ sys, int ld,
+                                                      int do_acc, int do_phi,
+                                                      const float* __restrict__ part_all,
+                                                      float* __restrict__ acc,
+                                                      float* __restrict__ phi, int accumulate) {
+  const khrr::SymSys S = sys[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S.n) return;
+  const size_t N = (size_t)S.n;
+  const float* part = part_all + S.part;
+  for (int c = 0; c < 4; ++c) {
+    if ((c < 3 && !do_acc) || (c == 3 && !do_phi)) continue;
+    float hi = 0.f, lo = 0.f;
+    for (int X = 0; X < S.n_sb; ++X) fold(hi, lo, part[((size_t)X * 4 + c) * N + i], COMP);
+    const float v = COMP ? hi + lo : hi;
+    const size_t g = (size_t)S.base + i;
+    if (c < 3) {
+      float* a = acc + (size_t)c * ld + g;
+      *a = accumulate ? *a + v : v;
+    } else {
+      phi[g] = accumulate ? phi[g] - v : -v;
     }
   }
 }
@@ -273,6 +325,23 @@ SymLauncher pick_sym_mode(bool a, bool p, bool c) {
   if (a && p) return c ? launch_sym<IPL, true, true, true> : launch_sym<IPL, true, true, false>;
   if (a) return c ? launch_sym<IPL, true, false, true> : launch_sym<IPL, true, false, false>;
   return c ? launch_sym<IPL, false, true, true> : launch_sym<IPL, false, true, false>;
+}
+
+template <bool A, bool P, bool C>
+void launch_sym_items(hipStream_t st, int n_items, const khrr::SymItem* items, int ld,
+                      const float* pos, const float* m, const float* h, const int* flags,
+                      float* part) {
+  hipLaunchKernelGGL((sym_items_kernel<8, A, P, C>), dim3(n_items), dim3(kSB / 8), 0, st, items, ld,
+                     pos, m, h, flags, part);
+}
+
+using SymItemsLauncher = void (*)(hipStream_t, int, const khrr::SymItem*, int, const float*,
+                                  const float*, const float*, const int*, float*);
+
+SymItemsLauncher pick_sym_items(bool a, bool p, bool c) {
+  if (a && p) return c ? launch_sym_items<true, true, true> : launch_sym_items<true, true, false>;
+  if (a) return c ? launch_sym_items<true, false, true> : launch_sym_items<true, false, false>;
+  return c ? launch_sym_items<false, true, true> : launch_sym_items<false, true, false>;
 }
 
 }  // namespace
@@ -322,6 +391,25 @@ int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const flo
     hipLaunchKernelGGL(sym_reduce<false>, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, n,
                        n_sb, (int)do_acc, (int)do_phi, part, acc, phi, accumulate);
   return check_launch(ctx, "sym_kernel");
+}
+
+int sym_items_launch(khrr_ctx* ctx, int n_items, const SymItem* items, int n_sys,
+                     const SymSys* sys, int max_n, int ld, const float* pos, const float* h,
+                     const float* m, const int* sb_flags, float* part, float* acc, float* phi,
+                     unsigned flags) {
+  if (n_items == 0) return 0;
+  const bool do_acc = acc != nullptr, do_phi = phi != nullptr;
+  const bool comp = (flags & KHRR_COMPENSATED) != 0;
+  const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
+  pick_sym_items(do_acc, do_phi, comp)(ctx->stream, n_items, items, ld, pos, m, h, sb_flags, part);
+  const dim3 grid(ceil_div(max_n, 256), n_sys);
+  if (comp)
+    hipLaunchKernelGGL(sym_reduce_sys<true>, grid, dim3(256), 0, ctx->stream, sys, ld, (int)do_acc,
+                       (int)do_phi, part, acc, phi, accumulate);
+  else
+    hipLaunchKernelGGL(sym_reduce_sys<false>, grid, dim3(256), 0, ctx->stream, sys, ld, (int)do_acc,
+                       (int)do_phi, part, acc, phi, accumulate);
+  return check_launch(ctx, "sym_items_kernel");
 }
 
 }  // namespace khrr

@@ -117,6 +117,7 @@ _SIGS = {
     "khrr_p2mesh_cell_phi": (_I, [_P, _I, ctypes.c_longlong, _P, _P, _P, _P, _P, _P, _I]),
     "khrr_server_create": (_I, [_P, ctypes.c_size_t, _I, ctypes.POINTER(_P)]),
     "khrr_server_destroy": (None, [_P]),
+    "khrr_server_set_round_pairs": (_I, [_P, _D]),
     "khrr_server_submit": (_I, [_P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_longlong)]),
     "khrr_server_step": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), _I, ctypes.POINTER(_I)]),
     "khrr_server_result": (_I, [_P, ctypes.c_longlong, ctypes.POINTER(_D), ctypes.POINTER(_D),
@@ -407,13 +408,21 @@ class Server:
     device-memory footprint, keeps their state on the device and advances
     all of them with one batched gravity launch per step."""
 
-    def __init__(self, ctx: Context, budget_bytes: int, max_active: int):
+    def __init__(self, ctx: Context, budget_bytes: int, max_active: int,
+                 round_pairs: float = 0.0):
         self._ctx = ctx
         self._lib = ctx._lib
         h = _P()
         ctx._check(self._lib.khrr_server_create(ctx._h, int(budget_bytes), int(max_active),
                                                 ctypes.byref(h)))
         self._h = h
+        if round_pairs:
+            self.set_round_pairs(round_pairs)
+
+    def set_round_pairs(self, pairs: float):
+        """0: every active simulation steps each round; > 0: least remaining
+        work first until a round holds `pairs` pair-interactions."""
+        self._ctx._check(self._lib.khrr_server_set_round_pairs(self._h, float(pairs)))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -1,8 +1,10 @@
 """The batched form (khrr_accel_batched): many independent systems of uneven N
 in one persistent launch.  Each system's result must equal the plain
-single-system launch (j_split min(16, ceil(n/16384)), j_tile 256) bit for
-bit, whatever else is in the batch, and must match the FP64 oracle within the gravity tolerance
-(median <= 1e-6, p99 <= 1e-5)."""
+single-system launch bit for bit, whatever else is in the batch: for
+n >= 32768 the pair-symmetric launch (symmetric 1, i_per_lane 8), below it
+the one-sided launch with j_split min(16, ceil(n/16384)), j_tile 256.  Results
+must also match the FP64 oracle within the gravity tolerance (median <= 1e-6,
+p99 <= 1e-5)."""
 import numpy as np
 import pytest
 
@@ -38,12 +40,14 @@ def _run_batched(ctx, off, pos, m, h, tune=None, phi=False, compensated=False):
     return acc.cpu().numpy(), (ph.cpu().numpy() if phi else None)
 
 
-def _pieces(n):
-    return max(1, min(16, -(-n // 16384)))
+def _plain_tune(n):
+    import khrr
+    if n >= 32768:
+        return khrr.Tune(8, 4, 1, 256, 1)
+    return khrr.Tune(4, 4, max(1, min(16, -(-n // 16384))), 256, 0)
 
 
 def _run_plain(ctx, pos, m, h, phi=False, compensated=False):
-    import khrr
     import torch
     n = pos.shape[1]
     acc = torch.zeros((3, n), device="cuda:0")
@@ -51,7 +55,7 @@ def _run_plain(ctx, pos, m, h, phi=False, compensated=False):
     p = to_dev(pos, torch.float32)
     hh = to_dev(h, torch.float32)
     ctx.accel(p, hh, p, to_dev(m, torch.float32), hh, acc=acc, phi=ph,
-              tune=khrr.Tune(4, 4, _pieces(n), 256), compensated=compensated)
+              tune=_plain_tune(n), compensated=compensated)
     torch.cuda.synchronize()
     return acc.cpu().numpy(), (ph.cpu().numpy() if phi else None)
 
@@ -117,10 +121,11 @@ def test_composition_change_and_empty_systems(ctx):
 
 
 @pytest.mark.parametrize("compensated", [False, True])
-def test_cut_systems_equal_plain_split(ctx, compensated):
-    # systems above 16384 bodies have their j-range cut into pieces summed
-    # in piece order; the plain launch with the same j_split agrees exactly
-    sizes = [40000, 1000, 16385, 70000, 16384]
+def test_large_systems_equal_plain(ctx, compensated):
+    # 16384 < n < 32768: j-range cut into pieces summed in piece order;
+    # n >= 32768: pair-symmetric superblock items; both agree exactly with
+    # the plain launch of the same shape
+    sizes = [40000, 1000, 16385, 70000, 16384, 32768, 32767]
     off, pos, m, h = _population(sizes, seed0=700)
     acc, phi = _run_batched(ctx, off, pos, m, h, phi=True, compensated=compensated)
     for s in range(len(sizes)):

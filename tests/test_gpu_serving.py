@@ -45,22 +45,30 @@ def _run_all(srv, reqs, arrivals):
     return done
 
 
-def test_alone_equals_in_the_mix(ctx):
+@pytest.mark.parametrize("kind", ["galaxy", "large"])
+def test_alone_equals_in_the_mix(ctx, kind):
     import khrr
-    target = dict(arrays=_galaxy_req(1024, 2048, 5), steps=12)
+    # "large" runs on the pair-symmetric path (n >= 32768), "galaxy" on the
+    # one-sided one; the others span both
+    if kind == "galaxy":
+        target = dict(arrays=_galaxy_req(1024, 2048, 5), steps=12)
+    else:
+        pos, vel, mass, eps = _plummer_req(36000, 6)
+        target = dict(arrays=(pos, vel, mass, np.full(36000, 0.004)), steps=6)
     others = [dict(arrays=_plummer_req(n, 40 + n), steps=s)
-              for n, s in ((4096, 20), (1024, 5), (2048, 9), (8192, 15))]
+              for n, s in ((4096, 20), (1024, 5), (2048, 9), (40960, 4), (8192, 15))]
     alone = khrr.Server(ctx, 1 << 30, 8)
     r_alone = _run_all(alone, [target], [0])[0]
     alone.close()
     mix = khrr.Server(ctx, 1 << 30, 8)
     reqs = others[:2] + [target] + others[2:]
-    r_mix = _run_all(mix, reqs, [0, 0, 3, 4, 6])[2]
+    r_mix = _run_all(mix, reqs, [0, 0, 3, 4, 5, 6])[2]
     mix.close()
     for a, b in zip(r_alone[:2], r_mix[:2]):
         assert a == b  # diagnostics dicts, exact
     np.testing.assert_array_equal(r_alone[2], r_mix[2])
-    assert abs(r_mix[2].sum() - (1.0 + 6.699356)) < 1e-9
+    total = 1.0 + 6.699356 if kind == "galaxy" else 1.0
+    assert abs(r_mix[2].sum() - total) < 1e-9
 
 
 def test_energy_follows_oracle(ctx):
@@ -112,4 +120,38 @@ def test_errors(ctx):
         srv.result(rid, 16 ** 3)
     with pytest.raises(khrr.KhrrError, match="unknown request"):
         srv.result(12345, 16 ** 3)
+    srv.close()
+
+
+def test_round_schedule_changes_timing_not_results(ctx):
+    import khrr
+    reqs = [dict(arrays=_plummer_req(n, 500 + n), steps=s)
+            for n, s in ((8192, 12), (1024, 30), (36864, 5), (2048, 8))]
+    runs = []
+    for pairs in (0.0, 1.0):  # lock-step; then least remaining work, one system per round
+        srv = khrr.Server(ctx, 1 << 30, 8, round_pairs=pairs)
+        ids = [srv.submit(*r["arrays"], 0.002, r["steps"], _grid()) for r in reqs]
+        order, res = [], {}
+        while len(res) < len(reqs):
+            for fid in srv.step():
+                order.append(ids.index(fid))
+                res[ids.index(fid)] = srv.result(fid, 16 ** 3)
+        st = srv.stats()
+        srv.close()
+        runs.append((order, res, st["rounds"]))
+    (o0, r0, n0), (o1, r1, n1) = runs
+    for k in range(len(reqs)):
+        assert r0[k][0] == r1[k][0] and r0[k][1] == r1[k][1]
+        np.testing.assert_array_equal(r0[k][2], r1[k][2])
+    # one system per round: every step is its own round, least work first
+    assert n1 == sum(r["steps"] for r in reqs) and n0 == max(r["steps"] for r in reqs)
+    work = [r["arrays"][0].shape[1] ** 2 * r["steps"] for r in reqs]
+    assert o1 == sorted(range(len(reqs)), key=lambda k: work[k])
+
+
+def test_round_pairs_rejects_negative(ctx):
+    import khrr
+    srv = khrr.Server(ctx, 1 << 26, 2)
+    with pytest.raises(khrr.KhrrError):
+        srv.set_round_pairs(-1.0)
     srv.close()
