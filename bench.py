@@ -484,10 +484,14 @@ def main():
              "dP_abs": math.sqrt(sum((d_end[k] - d_start[k]) ** 2 for k in ("Px", "Py", "Pz"))),
              "dLz_rel": d_end["Lz"] / d_start["Lz"] - 1.0,
              "E0": energy(d0)}
+    # HBM bytes per launch from the committed counter passes, when they were
+    # taken on the kernel this run uses
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         try:
-            traffic = json.load(open(TRAFFIC_FILE)).get("gravity_bytes_per_launch")
+            tf = json.load(open(TRAFFIC_FILE))
+            if tf.get("kernel", "").startswith("sym_kernel" if tune.symmetric else "grav_kernel"):
+                traffic = tf.get("gravity_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
     result = {
