@@ -94,3 +94,37 @@ def test_zero_stays_zero_and_rotation(ctx):
     p1, c1, n1 = ctx.wave_iterate(w, a, b, c, src, 1)
     assert (p1.data_ptr(), c1.data_ptr(), n1.data_ptr()) == (b.data_ptr(), c.data_ptr(),
                                                              a.data_ptr())
+
+
+@pytest.mark.parametrize("nx,iters", [(37, 5), (70, 4), (20, 2), (64, 7), (18, 6)])
+def test_two_step_pass_equals_single_steps(ctx, nx, iters):
+    # khrr_wave_iterate runs pairs of iterations as one two-step pass; the
+    # fields and the buffer roles must equal single steps with the
+    # reference rotation bit for bit
+    import khrr
+    import torch
+    w = khrr.reference_wave(nx, 4.0 / nx)
+    G = nx ** 3
+    rng = np.random.default_rng(nx * 10 + iters)
+    mass = rng.uniform(0, 1e-3, G)
+    src = torch.zeros(G, device="cuda:0")
+    ctx.wave_source(w, to_dev(mass, torch.float64), src)
+    init = [rng.normal(size=G).astype(np.float32) for _ in range(2)]
+    a, b, c = (torch.zeros(G, device="cuda:0") for _ in range(3))
+    a.copy_(to_dev(init[0], torch.float32))
+    b.copy_(to_dev(init[1], torch.float32))
+    p, cu, nxt = ctx.wave_iterate(w, a, b, c, src, iters)
+    torch.cuda.synchronize()
+    got_prev, got_curr = p.cpu().numpy().copy(), cu.cpu().numpy().copy()
+    roles = (p.data_ptr(), cu.data_ptr(), nxt.data_ptr())
+    a.copy_(to_dev(init[0], torch.float32))
+    b.copy_(to_dev(init[1], torch.float32))
+    c.zero_()
+    sp, sc, sn = a, b, c
+    for _ in range(iters):
+        ctx.wave_step(w, sn, sc, sp, src)
+        sp, sc, sn = sc, sn, sp
+    torch.cuda.synchronize()
+    assert roles == (sp.data_ptr(), sc.data_ptr(), sn.data_ptr())
+    np.testing.assert_array_equal(got_curr, sc.cpu().numpy())
+    np.testing.assert_array_equal(got_prev, sp.cpu().numpy())
