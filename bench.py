@@ -18,6 +18,10 @@ Also reported on the same line (one GPU):
   stencil       200 damped-wave iterations on that grid from phi = 0
   batched       512 Plummer spheres (N log-uniform 1024..131072) per step,
                 one persistent launch vs one plain launch per system
+  serving       a Poisson stream of mixed requests through the server
+                (two round schedules) vs the same requests one at a time
+  precision     3.1M-body two-galaxy encounter: plain vs compensated sums,
+                force error vs the FP64 oracle and drift over a few steps
   drift         E, P, L relative drift over the timed steps (FP64 device
                 reductions)
 
@@ -57,6 +61,8 @@ def parse():
     ap.add_argument("--batched-steps", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-lambda", type=float, default=3.0, help="arrivals per second")
+    ap.add_argument("--precision-steps", type=int, default=3,
+                    help="steps of the 3.1M-body encounter per sum mode (0: skip)")
     ap.add_argument("--shape", default=None,
                     help="fixed launch shape i_per_lane,waves_per_block,j_split,j_tile,symmetric "
                          "instead of the autotuner (for profiler passes)")
@@ -88,10 +94,11 @@ class Sim:
         self.phi = torch.zeros(n, **f32)
         self.t = 0.0
         self.tune = None  # launch shape; None = the context's tuned/static choice
+        self.compensated = False
 
     def gravity(self, out, phi=None):
         self.ctx.accel(self.pos32, self.h32, self.pos32, self.m32, self.h32, acc=out, phi=phi,
-                       tune=self.tune)
+                       tune=self.tune, compensated=self.compensated)
 
     def init_forces(self):
         self.gravity(self.acc)
@@ -107,12 +114,68 @@ class Sim:
 
     def diagnostics(self):
         self.ctx.accel(self.pos32, self.h32, self.pos32, self.m32, self.h32, phi=self.phi,
-                       tune=self.tune)
+                       tune=self.tune, compensated=self.compensated)
         return self.ctx.diag(self.pos, self.vel, self.mass64, self.phi)
 
 
 def energy(d):
     return 0.5 * (d["Ek"] + d["Ep"])
+
+
+def bench_precision(ctx, steps, samples=8192):
+    """Two-galaxy encounter (2 x (524288 + 1048576) = 3145728 bodies): plain
+    FP32 sums vs compensated (TwoSum) sums.  Force error against the FP64
+    oracle on `samples` random bodies at t = 0, and FP64 energy, momentum and
+    angular-momentum drift over `steps` predictor-corrector steps."""
+    import numpy as np
+    import torch
+    import ics
+    import khrr
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    pos, vel, mass, h = ics.encounter()
+    n = pos.shape[1]
+    rows = np.sort(np.random.default_rng(samples).choice(n, samples, replace=False))
+    m32 = mass.astype(np.float32).astype(np.float64)
+    h32 = h.astype(np.float32).astype(np.float64)
+    p32 = pos.astype(np.float32).astype(np.float64)
+    t0 = time.perf_counter()
+    ref, _ = orc.accel_rows(rows, p32, h32, m32)
+    t_orc = time.perf_counter() - t0
+    out = {"workload": "two-galaxy encounter, 2 x (524288 + 1048576) bodies, X = +-2, "
+                       "tilts 0/45 deg, V = (-+0.3, +-0.05)",
+           "n_bodies": n, "samples": samples, "steps": steps, "oracle_s": t_orc}
+    for mode in ("plain", "compensated"):
+        sim = Sim(ctx, pos, vel, mass, h, 0.002)
+        sim.compensated = mode == "compensated"
+        torch.cuda.synchronize()
+        ctx.reset_timing()
+        ctx.set_timing(True)
+        sim.init_forces()
+        torch.cuda.synchronize()
+        launches, ms = ctx.kernel_time(khrr.K_GRAVITY)
+        acc0 = sim.acc.cpu().numpy()[:, rows].astype(np.float64)
+        e = np.linalg.norm(acc0 - ref, axis=0) / np.linalg.norm(ref, axis=0)
+        d0 = sim.diagnostics()
+        for _ in range(steps):
+            sim.step()
+        d1 = sim.diagnostics()
+        ctx.set_timing(False)
+        torch.cuda.synchronize()
+        out[mode] = {
+            "force_ms": ms / max(launches, 1),
+            "Gpairs_per_s": float(n) * n / (ms / max(launches, 1) * 1e-3) / 1e9,
+            "err_median": float(np.median(e)), "err_p99": float(np.percentile(e, 99)),
+            "err_max": float(np.max(e)),
+            "dE_rel": energy(d1) / energy(d0) - 1.0,
+            "dP_abs": math.sqrt(sum((d1[k] - d0[k]) ** 2 for k in ("Px", "Py", "Pz"))),
+            "dLz_rel": d1["Lz"] / d0["Lz"] - 1.0}
+        log(f"precision {mode}: {out[mode]['force_ms']:.1f} ms per force evaluation, "
+            f"error median {out[mode]['err_median']:.2e} p99 {out[mode]['err_p99']:.2e}, "
+            f"dE/E {out[mode]['dE_rel']:.2e} over {steps} steps")
+        del sim
+        torch.cuda.empty_cache()
+    return out
 
 
 def bench_binning(ctx, sim, reps=5):
@@ -544,6 +607,10 @@ def main():
         if args.serve_requests > 0:
             log("serving")
             result["serving"] = bench_serving(ctx, args.serve_requests, args.serve_lambda)
+            torch.cuda.empty_cache()
+        if args.precision_steps > 0:
+            log("precision form")
+            result["precision"] = bench_precision(ctx, args.precision_steps)
     print(json.dumps(result), flush=True)
 
 

@@ -102,7 +102,8 @@ typedef struct {
   int symmetric;    /* 1: when the i-set and j-set are the same buffers,
                        evaluate each unordered pair once and apply it to
                        both bodies (needs i_per_lane 4 or 8 and a partial
-                       buffer of 16 B per body per 4096 bodies, at most
+                       buffer of 16 B per body per superblock -- 4096
+                       bodies from n = 131072 on, 2048 below -- at most
                        40 GiB; otherwise the one-sided kernel runs)      */
 } khrr_tune;
 
@@ -299,8 +300,8 @@ typedef struct {
 /* hbm_budget: device bytes the server may hold for simulation state: a
  * simulation of n bodies holds 152 B per body (FP32 gravity arrays and FP64
  * state) plus the batched gravity's partial sums for it (16 B per body per
- * 4096-body superblock when n >= 32768, 16 B per body per j-piece when
- * 16384 < n < 32768, else none).  The FP32 arena for budget / 152 bodies is
+ * superblock when n >= 32768 -- 2048 bodies below 131072, 4096 from there
+ * on -- 16 B per body per j-piece when 16384 < n < 32768, else none).  The FP32 arena for budget / 152 bodies is
  * reserved at creation.  max_active: simultaneous simulations (>= 1). */
 int khrr_server_create(khrr_ctx* ctx, size_t hbm_budget, int max_active, khrr_server** out);
 void khrr_server_destroy(khrr_server* srv);

@@ -218,12 +218,10 @@ BatchLauncher pick_batched(int ipl, bool a, bool p, bool c) {
   }
 }
 
-// Systems of at least this many bodies take the pair-symmetric path (eight
-// or more superblocks: the one-sided diagonal items are at most 1/8 of
-// the system's work).  Like the piece count, a rule of the system's size.
-inline bool on_sym_path(int n) {
-  return n >= 8 * khrr::kSymSuperblock && khrr::sym_eligible(n, 8);
-}
+// Systems of at least 32768 bodies take the pair-symmetric path (16 or
+// more superblocks: the one-sided diagonal items are at most 1/16 of the
+// system's work).  Like the piece count, a rule of the system's size.
+inline bool on_sym_path(int n) { return n >= 32768 && khrr::sym_eligible(n, 8); }
 
 // Device work list of one composition.
 struct WorkList {
@@ -236,6 +234,7 @@ struct WorkList {
   long long part_units = 0;  // sum of k*count over cut systems
   khrr::SymItem* sym_items = nullptr;
   int n_sym_items = 0;
+  int n_sym_big = 0;
   khrr::SymSys* sym_sys = nullptr;
   int n_sym_sys = 0;
   int max_sym_n = 0;
@@ -273,27 +272,30 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
   key[2 * (size_t)n_sys] = j_tile;
   const bool same = ctx->batch_ipl == ipl && ctx->batch_off == key;
   if (!same) {
-    constexpr int SB = khrr::kSymSuperblock;
     std::vector<int> flag_base(n_sys);
     std::vector<int2> tiles;
-    std::vector<khrr::SymItem> sym_items;
+    std::vector<khrr::SymItem> sym_big, sym_small;
     std::vector<khrr::SymSys> sym_sys;
     long long sym_floats = 0;
     int max_sym = 0;
     for (int s = 0; s < n_sys; ++s) {
       flag_base[s] = (int)tiles.size();
       const int b = host_begin[s], n = host_count[s], e = b + n;
-      const int step = on_sym_path(n) ? SB : j_tile;
+      const int SB = on_sym_path(n) ? khrr::sym_superblock(n) : 0;
+      const int step = SB ? SB : j_tile;
       for (int jt = b; jt < e; jt += step) tiles.push_back(make_int2(jt, std::min(e, jt + step)));
-      if (!on_sym_path(n)) continue;
+      if (!SB) continue;
       const int n_sb = (n + SB - 1) / SB;
+      auto& dst = SB == 4096 ? sym_big : sym_small;
       for (int A = 0; A < n_sb; ++A)
         for (int B = A; B < n_sb; ++B)
-          sym_items.push_back(khrr::SymItem{b, n, A, B, flag_base[s], 0, sym_floats});
+          dst.push_back(khrr::SymItem{b, n, A, B, flag_base[s], 0, sym_floats});
       sym_sys.push_back(khrr::SymSys{b, n, n_sb, 0, sym_floats});
       sym_floats += 4ll * n_sb * n;
       max_sym = std::max(max_sym, n);
     }
+    std::vector<khrr::SymItem> sym_items(sym_big);
+    sym_items.insert(sym_items.end(), sym_small.begin(), sym_small.end());
     std::vector<int> order(n_sys);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
@@ -361,6 +363,7 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
     ctx->batch_red_max = max_red;
     ctx->batch_units = unit;
     ctx->batch_sym_items = (int)sym_items.size();
+    ctx->batch_sym_big = (int)sym_big.size();
     ctx->batch_sym_sys = (int)sym_sys.size();
     ctx->batch_sym_max = max_sym;
     ctx->batch_sym_floats = sym_floats;
@@ -379,6 +382,7 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
   wl->max_red_count = ctx->batch_red_max;
   wl->part_units = ctx->batch_units;
   wl->n_sym_items = ctx->batch_sym_items;
+  wl->n_sym_big = ctx->batch_sym_big;
   wl->n_sym_sys = ctx->batch_sym_sys;
   wl->max_sym_n = ctx->batch_sym_max;
   wl->sym_floats = ctx->batch_sym_floats;
@@ -412,7 +416,8 @@ namespace khrr {
 size_t batched_part_bytes(int n, bool compensated) {
   if (n <= 0) return 0;
   if (on_sym_path(n)) {
-    const size_t n_sb = (size_t)(n + kSymSuperblock - 1) / kSymSuperblock;
+    const int sb = sym_superblock(n);
+    const size_t n_sb = (size_t)(n + sb - 1) / sb;
     return n_sb * 4 * (size_t)n * sizeof(float);
   }
   const int k = pieces_of(n);  // the piece count before tile rounding bounds it
@@ -444,9 +449,9 @@ int batched_segments(khrr_ctx* ctx, int n_sys, const int* host_begin, const int*
   const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
   Timed tm(ctx, KHRR_K_GRAVITY);
   // pair-symmetric systems first (one workgroup per (A, B) item) ...
-  KHRR_TRY(sym_items_launch(ctx, wl.n_sym_items, wl.sym_items, wl.n_sym_sys, wl.sym_sys,
-                            wl.max_sym_n, ld, pos, h, m, wl.tile_flags, sym_part, acc, phi,
-                            flags));
+  KHRR_TRY(sym_items_launch(ctx, wl.n_sym_big, wl.n_sym_items - wl.n_sym_big, wl.sym_items,
+                            wl.n_sym_sys, wl.sym_sys, wl.max_sym_n, ld, pos, h, m, wl.tile_flags,
+                            sym_part, acc, phi, flags));
   if (wl.n_items == 0) return 0;
   // ... then the persistent one-sided grid: as many blocks as stay resident,
   // never more waves than items

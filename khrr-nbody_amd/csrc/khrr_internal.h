@@ -16,9 +16,8 @@
 namespace khrr {
 
 constexpr int kWave = 64;  // CDNA wavefront width
-constexpr int kSymSuperblock = 4096;  // bodies per superblock of the pair-symmetric form
 
-// Pair-symmetric batched form: one (A, B) superblock item of the system
+// Pair-symmetric form: one (A, B) superblock item of the system
 // whose bodies start at `base`; its superblock flags start at
 // flags[flag_base] and its partial slots at part[part] (n_sb*4*n floats).
 struct SymItem {
@@ -81,6 +80,7 @@ struct khrr_ctx {
   int batch_red_max = 0;      // largest such system
   long long batch_units = 0;  // partial-sum rows of all cut systems
   int batch_sym_items = 0;    // pair-symmetric (A, B) items
+  int batch_sym_big = 0;      // of which the first ones have 4096-body superblocks
   int batch_sym_sys = 0;      // systems on the pair-symmetric path
   int batch_sym_max = 0;      // largest such system
   long long batch_sym_floats = 0;  // their partial slots
@@ -163,13 +163,17 @@ int batched_segments(khrr_ctx* ctx, int n_sys, const int* host_begin, const int*
 // of n bodies (0 when it writes its outputs directly).
 size_t batched_part_bytes(int n, bool compensated);
 // Pair-symmetric form (khrr_sym.hip) for one system (i-set == j-set).
+// Bodies per superblock for a system of n bodies: 4096 from 131072 bodies
+// on, else 2048 (a rule of n alone, shared by the plain and batched forms).
+int sym_superblock(int n);
 bool sym_eligible(int n, int ipl);
 bool sym_worthwhile(const khrr_ctx* ctx, int n);
 int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const float* m, float* acc,
                float* phi, unsigned flags, int ipl);
-// Batched pair-symmetric launch (i_per_lane 8) over device item and system
-// tables, then the per-system slot reduction; max_n bounds every system.
-int sym_items_launch(khrr_ctx* ctx, int n_items, const SymItem* items, int n_sys,
+// Batched pair-symmetric launches (i_per_lane 8) over a device item table
+// holding n_big items of 4096-body superblocks then n_small of 2048, then
+// the per-system slot reduction; max_n bounds every system.
+int sym_items_launch(khrr_ctx* ctx, int n_big, int n_small, const SymItem* items, int n_sys,
                      const SymSys* sys, int max_n, int ld, const float* pos, const float* h,
                      const float* m, const int* sb_flags, float* part, float* acc, float* phi,
                      unsigned flags);

@@ -202,76 +202,72 @@ __global__ __launch_bounds__(256) void block_bounds(int n, int nb, const int* __
   if (b <= nb) bounds[b] = lower_bound_dev(skeys, n, (long long)b * kCellBlock);
 }
 
-// One 512-cell block of the reference per loop trip; a resident grid walks
-// the blocks with a stride and loads the next block's bounds while it
-// writes the current one (most blocks of a fine grid are empty and are just
-// four coalesced stores per cell).
+// One workgroup per 512-cell block of the reference.  Empty blocks (most
+// of a fine grid) are four coalesced stores per cell; in the others each
+// cell walks its sorted segment in index order, eight particles per trip
+// (their indices and masses are loaded before the in-order FP64 adds).
 __global__ __launch_bounds__(kCellBlock) void bin_cells(
-    int n, long long G, int cblocks, const int* __restrict__ skeys, const int* __restrict__ svals,
+    int n, long long G, const int* __restrict__ skeys, const int* __restrict__ svals,
     const int* __restrict__ bounds, const double* __restrict__ mass, int* __restrict__ count,
     int* __restrict__ offset, int* __restrict__ start_ref, int* __restrict__ block_incl,
     int* __restrict__ id_in_cell, double* __restrict__ cell_mass) {
   __shared__ int cnt[kCellBlock];
   __shared__ int wsum[kCellBlock / khrr::kWave];
   const int tid = threadIdx.x;
+  const long long c0 = (long long)blockIdx.x * kCellBlock;
+  const long long c = c0 + tid;
+  const int lo = bounds[blockIdx.x], hi = bounds[blockIdx.x + 1];
+  if (tid == 0 && block_incl) block_incl[blockIdx.x] = hi;
+  if (lo == hi) {
+    if (c < G) {
+      count[c] = 0;
+      if (offset) offset[c] = lo;
+      if (start_ref) start_ref[c] = lo;
+      if (cell_mass) cell_mass[c] = 0.0;
+    }
+    return;
+  }
+  cnt[tid] = 0;
+  __syncthreads();
+  for (int p = lo + tid; p < hi; p += kCellBlock) atomicAdd(&cnt[skeys[p] - (int)c0], 1);
+  __syncthreads();
+  // Exclusive scan of cnt: wave-inclusive scans, then wave totals.
   const int lane = tid & (khrr::kWave - 1);
   const int wave = tid / khrr::kWave;
-  int b = blockIdx.x;
-  if (b >= cblocks) return;
-  int lo = bounds[b], hi = bounds[b + 1];
-  while (b < cblocks) {
-    const int nb = b + (int)gridDim.x;
-    int nlo = 0, nhi = 0;
-    if (nb < cblocks) {
-      nlo = bounds[nb];
-      nhi = bounds[nb + 1];
-    }
-    const long long c0 = (long long)b * kCellBlock;
-    const long long c = c0 + tid;
-    if (tid == 0 && block_incl) block_incl[b] = hi;
-    if (lo == hi) {
-      // empty block (lo, hi are the same for every thread)
-      if (c < G) {
-        count[c] = 0;
-        if (offset) offset[c] = lo;
-        if (start_ref) start_ref[c] = lo;
-        if (cell_mass) cell_mass[c] = 0.0;
-      }
-    } else {
-      cnt[tid] = 0;
-      __syncthreads();
-      for (int p = lo + tid; p < hi; p += kCellBlock) atomicAdd(&cnt[skeys[p] - (int)c0], 1);
-      __syncthreads();
-      // Exclusive scan of cnt: wave-inclusive scans, then wave totals.
-      const int mine = cnt[tid];
-      int incl = mine;
+  const int mine = cnt[tid];
+  int incl = mine;
 #pragma unroll
-      for (int off = 1; off < khrr::kWave; off <<= 1) {
-        const int v = __shfl_up(incl, off, khrr::kWave);
-        if (lane >= off) incl += v;
-      }
-      if (lane == khrr::kWave - 1) wsum[wave] = incl;
-      __syncthreads();
-      int before = 0;
-      for (int w = 0; w < wave; ++w) before += wsum[w];
-      const int excl = lo + before + incl - mine;
-      if (c < G) {
-        count[c] = mine;
-        if (offset) offset[c] = excl;
-        if (start_ref) start_ref[c] = lo;
-        double m = 0.0;
-        for (int k = 0; k < mine; ++k) {
-          const int v = svals[excl + k];
-          if (id_in_cell) id_in_cell[v] = k;
-          if (cell_mass) m += mass[v];
-        }
-        if (cell_mass) cell_mass[c] = m;
+  for (int off = 1; off < khrr::kWave; off <<= 1) {
+    const int v = __shfl_up(incl, off, khrr::kWave);
+    if (lane >= off) incl += v;
+  }
+  if (lane == khrr::kWave - 1) wsum[wave] = incl;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += wsum[w];
+  const int excl = lo + before + incl - mine;
+  if (c >= G) return;
+  count[c] = mine;
+  if (offset) offset[c] = excl;
+  if (start_ref) start_ref[c] = lo;
+  double m = 0.0;
+  constexpr int kB = 8;
+  for (int k0 = 0; k0 < mine; k0 += kB) {
+    int v[kB];
+    double mv[kB];
+#pragma unroll
+    for (int j = 0; j < kB; ++j) v[j] = k0 + j < mine ? svals[excl + k0 + j] : 0;
+#pragma unroll
+    for (int j = 0; j < kB; ++j) mv[j] = (cell_mass && k0 + j < mine) ? mass[v[j]] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kB; ++j) {
+      if (k0 + j < mine) {
+        if (id_in_cell) id_in_cell[v[j]] = k0 + j;
+        m += mv[j];
       }
     }
-    b = nb;
-    lo = nlo;
-    hi = nhi;
   }
+  if (cell_mass) cell_mass[c] = m;
 }
 
 // perm[offset[cell_id[i]] + id_in_cell[i]] = i: the sorted segments back
@@ -420,9 +416,8 @@ extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, cons
   }
   hipLaunchKernelGGL(block_bounds, dim3(ceil_div(cblocks + 1, 256)), dim3(256), 0, ctx->stream, n,
                      cblocks, skeys, bounds);
-  const int cgrid = (int)std::min<long long>(cblocks, 8ll * ctx->cu_count);
-  hipLaunchKernelGGL(bin_cells, dim3(cgrid), dim3(kCellBlock), 0, ctx->stream, n, G, cblocks,
-                     skeys, svals, bounds, mass, count, offset, start_id_ref, block_incl,
-                     id_in_cell, cell_mass);
+  hipLaunchKernelGGL(bin_cells, dim3(cblocks), dim3(kCellBlock), 0, ctx->stream, n, G, skeys,
+                     svals, bounds, mass, count, offset, start_id_ref, block_incl, id_in_cell,
+                     cell_mass);
   return check_launch(ctx, "p2mesh kernels");
 }

@@ -7,8 +7,10 @@
 // transcendental, subtraction and r^2 work per directed pair.
 //
 // Work decomposition (gfx950):
-//  * bodies are cut into superblocks of kSB = 4096; a workgroup takes one
-//    item (A, B) of the upper triangle, A <= B;
+//  * bodies are cut into superblocks of SB bodies (4096 for n >= 131072,
+//    else 2048, so mid-size systems still make enough items); a workgroup
+//    of SB / (64 IPL) waves takes one item (A, B) of the upper triangle,
+//    A <= B;
 //  * off-diagonal items: each wave owns 64*IPL i-bodies of A; the j-bodies of
 //    B stream through in chunks of 64, one per lane, and rotate one lane per
 //    step (DPP wave_ror:1) together with their accumulators, so after 64
@@ -32,7 +34,6 @@ using khrr::fold;
 using khrr::kUniformH;
 using khrr::kUniformM;
 
-constexpr int kSB = khrr::kSymSuperblock;
 
 __device__ __forceinline__ float ror1(float v) {
   // DPP wave_ror:1 (dpp_ctrl 0x13C): every lane receives its neighbour's
@@ -103,13 +104,14 @@ __device__ __forceinline__ void chunk64(float& xj, float& yj, float& zj, float& 
 // flags and part its slot block, laid out part[(X*4 + c)*n + i] with
 // c = ax, ay, az (acceleration from the bodies of superblock X) and
 // 3 = sum_j m_j / r (positive).
-template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
+template <int IPL, int W, bool DO_ACC, bool DO_PHI, bool COMP>
 __device__ __forceinline__ void sym_block(
     int n, int A, int B, const float* __restrict__ xs, const float* __restrict__ ys,
     const float* __restrict__ zs, const float* __restrict__ m, const float* __restrict__ h,
     const int* __restrict__ sb_flags, float* __restrict__ part,
-    float (&red)[kSB / (khrr::kWave * IPL)][4][khrr::kWave]) {
-  constexpr int W = kSB / (khrr::kWave * IPL);
+    float (&red)[W][4][khrr::kWave]) {
+  static_assert(W >= 4, "the j-side sum uses 4 x 64 threads");
+  constexpr int SB = khrr::kWave * IPL * W;  // bodies per superblock
   const int lane = threadIdx.x & (khrr::kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / khrr::kWave);
   const size_t N = (size_t)n;
@@ -119,7 +121,7 @@ __device__ __forceinline__ void sym_block(
   int idx[IPL];
 #pragma unroll
   for (int k = 0; k < IPL; ++k) {
-    idx[k] = A * kSB + wave * (khrr::kWave * IPL) + k * khrr::kWave + lane;
+    idx[k] = A * SB + wave * (khrr::kWave * IPL) + k * khrr::kWave + lane;
     const bool ok = idx[k] < n;
     // an absent i sits at 1e30: r^2 overflows to inf, rsq gives 0 and every
     // term it adds to a j-body is exactly 0
@@ -134,16 +136,16 @@ __device__ __forceinline__ void sym_block(
 
   if (A == B) {
     // diagonal: one-sided over the superblock's own bodies (j = i included)
-    const int j0 = A * kSB, j1 = min(n, j0 + kSB);
+    const int j0 = A * SB, j1 = min(n, j0 + SB);
     for (int jt = j0; jt < j1; jt += 256)
       khrr::tile_step<IPL, DO_ACC, DO_PHI, COMP>(jt, min(j1, jt + 256), sb_flags[A], xs, ys, zs,
                                                  m, h, xi, yi, zi, hi, acc);
   } else {
     const bool uh = (flags & kUniformH) != 0;
     const bool um = (flags & kUniformM) != 0;
-    const int j0 = B * kSB, j1 = min(n, j0 + kSB);
-    const float hsum = h[A * kSB] + h[j0];
-    const float mA = m[A * kSB], mB = m[j0];
+    const int j0 = B * SB, j1 = min(n, j0 + SB);
+    const float hsum = h[A * SB] + h[j0];
+    const float mA = m[A * SB], mB = m[j0];
     float hh[IPL];  // h_i + h_j of the uniform path
 #pragma unroll
     for (int k = 0; k < IPL; ++k) hh[k] = hsum;
@@ -206,14 +208,14 @@ __device__ __forceinline__ void sym_block(
   }
 }
 
-template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
-__global__ __launch_bounds__(kSB / IPL) void sym_kernel(int n, int n_sb,
-                                                        const float* __restrict__ pos,
-                                                        const float* __restrict__ m,
-                                                        const float* __restrict__ h,
-                                                        const int* __restrict__ sb_flags,
-                                                        float* __restrict__ part) {
-  __shared__ float red[kSB / (khrr::kWave * IPL)][4][khrr::kWave];
+template <int IPL, int W, bool DO_ACC, bool DO_PHI, bool COMP>
+__global__ __launch_bounds__(khrr::kWave* W) void sym_kernel(int n, int n_sb,
+                                                             const float* __restrict__ pos,
+                                                             const float* __restrict__ m,
+                                                             const float* __restrict__ h,
+                                                             const int* __restrict__ sb_flags,
+                                                             float* __restrict__ part) {
+  __shared__ float red[W][4][khrr::kWave];
   // item -> (A, B), row-major over the upper triangle
   int t = blockIdx.x, A = 0, row = n_sb;
   while (t >= row) {
@@ -222,23 +224,24 @@ __global__ __launch_bounds__(kSB / IPL) void sym_kernel(int n, int n_sb,
     --row;
   }
   const size_t N = (size_t)n;
-  sym_block<IPL, DO_ACC, DO_PHI, COMP>(n, A, A + t, pos, pos + N, pos + 2 * N, m, h, sb_flags,
-                                       part, red);
+  sym_block<IPL, W, DO_ACC, DO_PHI, COMP>(n, A, A + t, pos, pos + N, pos + 2 * N, m, h, sb_flags,
+                                          part, red);
 }
 
 // Batched form: workgroup b takes items[b], an (A, B) item of one system of
-// a segment table (planes of stride ld).
-template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
-__global__ __launch_bounds__(kSB / IPL) void sym_items_kernel(
+// a segment table (planes of stride ld); every item of one launch has the
+// superblock size 64 * 8 * W.
+template <int W, bool DO_ACC, bool DO_PHI, bool COMP>
+__global__ __launch_bounds__(khrr::kWave* W) void sym_items_kernel(
     const khrr::SymItem* __restrict__ items, int ld, const float* __restrict__ pos,
     const float* __restrict__ m, const float* __restrict__ h, const int* __restrict__ flags,
     float* __restrict__ part) {
-  __shared__ float red[kSB / (khrr::kWave * IPL)][4][khrr::kWave];
+  __shared__ float red[W][4][khrr::kWave];
   const khrr::SymItem it = items[blockIdx.x];
   const size_t b = (size_t)it.base;
-  sym_block<IPL, DO_ACC, DO_PHI, COMP>(it.n, it.A, it.B, pos + b, pos + ld + b,
-                                       pos + 2 * (size_t)ld + b, m + b, h + b,
-                                       flags + it.flag_base, part + it.part, red);
+  sym_block<8, W, DO_ACC, DO_PHI, COMP>(it.n, it.A, it.B, pos + b, pos + ld + b,
+                                        pos + 2 * (size_t)ld + b, m + b, h + b,
+                                        flags + it.flag_base, part + it.part, red);
 }
 
 // acc[i] (+)= sum_X part[X][0..2][i];  phi[i] (-)= sum_X part[X][3][i]
@@ -292,13 +295,14 @@ __global__ __launch_bounds__(256) void sym_reduce_sys(const khrr::SymSys* __rest
 }
 
 // sb_flags[X]: kUniformH / kUniformM over superblock X.
-__global__ __launch_bounds__(256) void sb_flags_kernel(int n, int n_sb, const float* __restrict__ h,
+__global__ __launch_bounds__(256) void sb_flags_kernel(int n, int n_sb, int sb,
+                                                       const float* __restrict__ h,
                                                        const float* __restrict__ m,
                                                        int* __restrict__ flags) {
   const int lane = threadIdx.x & (khrr::kWave - 1);
   const int X = blockIdx.x * (blockDim.x / khrr::kWave) + threadIdx.x / khrr::kWave;
   if (X >= n_sb) return;
-  const int b = X * kSB, e = min(n, b + kSB);
+  const int b = X * sb, e = min(n, b + sb);
   const unsigned href = __float_as_uint(h[b]), mref = __float_as_uint(m[b]);
   bool sh = true, sm = true;
   for (int j = b + lane; j < e; j += khrr::kWave) {
@@ -309,39 +313,45 @@ __global__ __launch_bounds__(256) void sb_flags_kernel(int n, int n_sb, const fl
   if (lane == 0) flags[X] = (bh == 0ull ? kUniformH : 0) | (bm == 0ull ? kUniformM : 0);
 }
 
-template <int IPL, bool A, bool P, bool C>
+template <int IPL, int W, bool A, bool P, bool C>
 void launch_sym(hipStream_t st, int n, int n_sb, const float* pos, const float* m, const float* h,
                 const int* flags, float* part) {
   const unsigned items = (unsigned)((long long)n_sb * (n_sb + 1) / 2);
-  hipLaunchKernelGGL((sym_kernel<IPL, A, P, C>), dim3(items), dim3(kSB / IPL), 0, st, n, n_sb, pos,
-                     m, h, flags, part);
+  hipLaunchKernelGGL((sym_kernel<IPL, W, A, P, C>), dim3(items), dim3(khrr::kWave * W), 0, st, n,
+                     n_sb, pos, m, h, flags, part);
 }
 
 using SymLauncher = void (*)(hipStream_t, int, int, const float*, const float*, const float*,
                              const int*, float*);
 
-template <int IPL>
+template <int IPL, int W>
 SymLauncher pick_sym_mode(bool a, bool p, bool c) {
-  if (a && p) return c ? launch_sym<IPL, true, true, true> : launch_sym<IPL, true, true, false>;
-  if (a) return c ? launch_sym<IPL, true, false, true> : launch_sym<IPL, true, false, false>;
-  return c ? launch_sym<IPL, false, true, true> : launch_sym<IPL, false, true, false>;
+  if (a && p) return c ? launch_sym<IPL, W, true, true, true> : launch_sym<IPL, W, true, true, false>;
+  if (a) return c ? launch_sym<IPL, W, true, false, true> : launch_sym<IPL, W, true, false, false>;
+  return c ? launch_sym<IPL, W, false, true, true> : launch_sym<IPL, W, false, true, false>;
 }
 
-template <bool A, bool P, bool C>
+SymLauncher pick_sym(int ipl, int sb, bool a, bool p, bool c) {
+  if (ipl == 8) return sb == 4096 ? pick_sym_mode<8, 8>(a, p, c) : pick_sym_mode<8, 4>(a, p, c);
+  return sb == 4096 ? pick_sym_mode<4, 16>(a, p, c) : pick_sym_mode<4, 8>(a, p, c);
+}
+
+template <int W, bool A, bool P, bool C>
 void launch_sym_items(hipStream_t st, int n_items, const khrr::SymItem* items, int ld,
                       const float* pos, const float* m, const float* h, const int* flags,
                       float* part) {
-  hipLaunchKernelGGL((sym_items_kernel<8, A, P, C>), dim3(n_items), dim3(kSB / 8), 0, st, items, ld,
-                     pos, m, h, flags, part);
+  hipLaunchKernelGGL((sym_items_kernel<W, A, P, C>), dim3(n_items), dim3(khrr::kWave * W), 0, st,
+                     items, ld, pos, m, h, flags, part);
 }
 
 using SymItemsLauncher = void (*)(hipStream_t, int, const khrr::SymItem*, int, const float*,
                                   const float*, const float*, const int*, float*);
 
-SymItemsLauncher pick_sym_items(bool a, bool p, bool c) {
-  if (a && p) return c ? launch_sym_items<true, true, true> : launch_sym_items<true, true, false>;
-  if (a) return c ? launch_sym_items<true, false, true> : launch_sym_items<true, false, false>;
-  return c ? launch_sym_items<false, true, true> : launch_sym_items<false, true, false>;
+template <int W>
+SymItemsLauncher pick_sym_items_mode(bool a, bool p, bool c) {
+  if (a && p) return c ? launch_sym_items<W, true, true, true> : launch_sym_items<W, true, true, false>;
+  if (a) return c ? launch_sym_items<W, true, false, true> : launch_sym_items<W, true, false, false>;
+  return c ? launch_sym_items<W, false, true, true> : launch_sym_items<W, false, true, false>;
 }
 
 }  // namespace
@@ -352,16 +362,20 @@ namespace khrr {
 // 786432 bodies and 38.6 GB at 3145728 (of 288 GB HBM).
 constexpr size_t kSymPartBudget = size_t(40) << 30;
 
+int sym_superblock(int n) { return n >= 131072 ? 4096 : 2048; }
+
 bool sym_eligible(int n, int ipl) {
   if (ipl != 4 && ipl != 8) return false;
-  const long long n_sb = (n + kSB - 1) / kSB;
+  const int sb = sym_superblock(n);
+  const long long n_sb = (n + (long long)sb - 1) / sb;
   if (n_sb < 2) return false;
   return (size_t)n_sb * 4 * (size_t)n * sizeof(float) <= kSymPartBudget;
 }
 
 // Enough (A, B) items to fill the device (about two workgroups per CU).
 bool sym_worthwhile(const khrr_ctx* ctx, int n) {
-  const long long n_sb = (n + kSB - 1) / kSB;
+  const int sb = sym_superblock(n);
+  const long long n_sb = (n + (long long)sb - 1) / sb;
   return n_sb * (n_sb + 1) / 2 >= 2ll * ctx->cu_count;
 }
 
@@ -370,7 +384,8 @@ int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const flo
   const bool do_acc = acc != nullptr, do_phi = phi != nullptr;
   const bool comp = (flags & KHRR_COMPENSATED) != 0;
   const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
-  const int n_sb = (n + kSB - 1) / kSB;
+  const int sb = sym_superblock(n);
+  const int n_sb = (n + sb - 1) / sb;
   const size_t part_elems = (size_t)n_sb * 4 * n;
   void* base = nullptr;
   KHRR_TRY(scratch(ctx, Carver::need(n_sb, sizeof(int)) + Carver::need(part_elems, sizeof(float)),
@@ -378,11 +393,10 @@ int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const flo
   Carver cv(base);
   int* sbf = cv.take<int>(n_sb);
   float* part = cv.take<float>(part_elems);
-  SymLauncher fn = ipl == 8 ? pick_sym_mode<8>(do_acc, do_phi, comp)
-                            : pick_sym_mode<4>(do_acc, do_phi, comp);
+  SymLauncher fn = pick_sym(ipl, sb, do_acc, do_phi, comp);
   Timed tm(ctx, KHRR_K_GRAVITY);
-  hipLaunchKernelGGL(sb_flags_kernel, dim3(ceil_div(n_sb, 4)), dim3(256), 0, ctx->stream, n, n_sb, h,
-                     m, sbf);
+  hipLaunchKernelGGL(sb_flags_kernel, dim3(ceil_div(n_sb, 4)), dim3(256), 0, ctx->stream, n, n_sb,
+                     sb, h, m, sbf);
   fn(ctx->stream, n, n_sb, pos, m, h, sbf, part);
   if (comp)
     hipLaunchKernelGGL(sym_reduce<true>, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, n,
@@ -393,15 +407,20 @@ int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const flo
   return check_launch(ctx, "sym_kernel");
 }
 
-int sym_items_launch(khrr_ctx* ctx, int n_items, const SymItem* items, int n_sys,
+int sym_items_launch(khrr_ctx* ctx, int n_big, int n_small, const SymItem* items, int n_sys,
                      const SymSys* sys, int max_n, int ld, const float* pos, const float* h,
                      const float* m, const int* sb_flags, float* part, float* acc, float* phi,
                      unsigned flags) {
-  if (n_items == 0) return 0;
+  if (n_big + n_small == 0) return 0;
   const bool do_acc = acc != nullptr, do_phi = phi != nullptr;
   const bool comp = (flags & KHRR_COMPENSATED) != 0;
   const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
-  pick_sym_items(do_acc, do_phi, comp)(ctx->stream, n_items, items, ld, pos, m, h, sb_flags, part);
+  if (n_big > 0)
+    pick_sym_items_mode<8>(do_acc, do_phi, comp)(ctx->stream, n_big, items, ld, pos, m, h, sb_flags,
+                                                 part);
+  if (n_small > 0)
+    pick_sym_items_mode<4>(do_acc, do_phi, comp)(ctx->stream, n_small, items + n_big, ld, pos, m, h,
+                                                 sb_flags, part);
   const dim3 grid(ceil_div(max_n, 256), n_sys);
   if (comp)
     hipLaunchKernelGGL(sym_reduce_sys<true>, grid, dim3(256), 0, ctx->stream, sys, ld, (int)do_acc,

@@ -102,6 +102,40 @@ def system_arrays(pos_s, vel_s, pos_dm, vel_dm, m_s, m_dm, eps_s, eps_d):
     return pos, vel, mass, h
 
 
+def place(pos, vel, center, vcenter, alpha_deg):
+    """A galaxy's rotation about y by alpha and its translation, applied in
+    place as the reference's loader does: the z update reads the already
+    rotated x (and vz the rotated vx), nbody.cu:1344-1358."""
+    ca, sa = np.cos(np.radians(alpha_deg)), np.sin(np.radians(alpha_deg))
+    pos = np.array(pos, dtype=np.float64)
+    vel = np.array(vel, dtype=np.float64)
+    pos[0] = center[0] + pos[0] * ca + pos[2] * sa
+    pos[1] += center[1]
+    pos[2] = center[2] + pos[2] * ca - pos[0] * sa
+    vel[0] = vcenter[0] + vel[0] * ca + vel[2] * sa
+    vel[1] += vcenter[1]
+    vel[2] = vcenter[2] + vel[2] * ca - vel[0] * sa
+    return pos, vel
+
+
+def encounter(n_s=524288, n_dm=1048576, eps=0.004, seeds=(12346, 12347)):
+    """Two disk + halo galaxies at X = +2 (tilt 0, V = (-0.3, +0.05, 0)) and
+    X = -2 (tilt 45 degrees, V = (+0.3, -0.05, 0)).  Particle order as the
+    reference loads them: stars of both galaxies, then their halos.
+    Returns pos, vel, mass, h."""
+    setup = (((2.0, 0.0, 0.0), (-0.3, 0.05, 0.0), 0.0), ((-2.0, 0.0, 0.0), (0.3, -0.05, 0.0), 45.0))
+    stars, halos = [], []
+    for seed, (c, v, a) in zip(seeds, setup):
+        ps, vs, pd, vd = galaxy(n_s, n_dm, seed=seed)
+        stars.append(place(ps, vs, c, v, a))
+        halos.append(place(pd, vd, c, v, a))
+    pos = np.concatenate([p for p, _ in stars] + [p for p, _ in halos], axis=1)
+    vel = np.concatenate([v for _, v in stars] + [v for _, v in halos], axis=1)
+    mass = np.concatenate([np.full(2 * n_s, 1.0 / n_s), np.full(2 * n_dm, 6.699356 / n_dm)])
+    h = np.full(pos.shape[1], 0.5 * eps * eps)
+    return pos, vel, mass, h
+
+
 def write_start(path, pos, vel, t=0.0):
     """Reference text format: `N t` then `x y z vx vy vz` lines (6 decimals
     as in the shipped start_S0.txt)."""
