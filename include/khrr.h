@@ -264,6 +264,25 @@ int khrr_wave_step(khrr_ctx* ctx, const khrr_wave* w, float* phi_new, const floa
 int khrr_wave_iterate(khrr_ctx* ctx, const khrr_wave* w, float** prev, float** curr,
                       float** next, const float* src, int iters);
 
+/* ---- mesh force of the wave solver ------------------------------------ */
+/* acc_field (p2mesh-kernel.cuh:169-224): cell_acc = -grad(cell_phi) of an
+ * nx^3 field (x fastest): 0.5 (phi[+1] - phi[-1]) inside, one-sided
+ * differences on the faces, times -1/dx; evaluated in FP64 from the FP32
+ * field and rounded to FP32.  cell_acc: vec3 planes of nx^3 floats;
+ * cell_acc_abs: nx^3 floats (|a|, norm3 of common.cuh:94) or NULL.  The
+ * reference launches this kernel over a 1-D grid (nbody.cu:244), which
+ * evaluates every cell as if iy = iz = 0 and reads past the field; here
+ * every cell gets its own 3-D stencil.  nx >= 2. */
+int khrr_mesh_acc(khrr_ctx* ctx, int nx, double dx, const float* cell_phi, float* cell_acc,
+                  float* cell_acc_abs);
+/* apply_acceleration_to_particles + apply_cell_to_particles
+ * (p2mesh-kernel.cuh:226-252): acc[.][i] = cell_acc[.][cell_id[i]] and
+ * phi[i] = cell_phi[cell_id[i]] (cell_id from khrr_p2mesh_bin; an index
+ * outside [0, n_cells) gives NaN).  acc: vec3 planes of n floats or NULL;
+ * phi: n floats or NULL (not both). */
+int khrr_mesh_to_particles(khrr_ctx* ctx, int n, long long n_cells, const int* cell_id,
+                           const float* cell_acc, const float* cell_phi, float* acc, float* phi);
+
 /* ---- serving form: continuous batching ------------------------------- */
 /* No counterpart in the reference (one simulation per process).  A server
  * keeps every admitted simulation's state resident in device memory and

@@ -22,6 +22,13 @@
 //     "S_%5d.bin" and never reads DM, :1251,1274);
 //   * the cell-potential buffer is accumulated across saves as in the
 //     reference (never cleared, :1628-1634); --clear-cell-phi clears it.
+//
+// --solver wave selects the reference's other force (NBodySolver::Wave,
+// :1128-1143, hard-wired off there): calc_acceleration_by_grid (:183-285)
+// bins the bodies, advances the damped-wave field (20000 iterations at
+// setup, dtgrav/dt_wave per step, the field kept between calls), and reads
+// -grad(phi) and phi at each body's cell.  The save block then skips PHI
+// and the cell-potential sum and writes the wave field (:1601-1666).
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -48,12 +55,15 @@ struct Options {
   int grid_nx = 200;
   double grid_dx = 0.2;
   bool json = false;
+  bool wave = false;
+  int setup_iters = 20000;  // _iterations_to_setup, nbody.cu:1143
 };
 
 void usage() {
   std::fprintf(stderr,
                "usage: khrr_nbody [--data DIR] [--compat-accumulate] [--compensated]\n"
-               "                  [--clear-cell-phi] [--max-steps N] [--grid NX DX] [--json]\n");
+               "                  [--clear-cell-phi] [--max-steps N] [--grid NX DX] [--json]\n"
+               "                  [--solver nbody|wave] [--setup-iters N]\n");
 }
 
 bool parse(int argc, char** argv, Options& o) {
@@ -74,6 +84,16 @@ bool parse(int argc, char** argv, Options& o) {
       o.grid_dx = std::atof(argv[++i]);
     } else if (a == "--json") {
       o.json = true;
+    } else if (a == "--solver" && i + 1 < argc) {
+      const std::string v = argv[++i];
+      if (v != "nbody" && v != "wave") {
+        usage();
+        return false;
+      }
+      o.wave = v == "wave";
+    } else if (a == "--setup-iters" && i + 1 < argc) {
+      o.setup_iters = std::atoi(argv[++i]);
+      if (o.setup_iters < 0) return false;
     } else {
       usage();
       return false;
@@ -274,6 +294,20 @@ int run(const Options& opt) {
   int* d_off = db.alloc<int>(G);
   double* d_cmass = db.alloc<double>(G);
   double* d_cphi = db.alloc<double>(G);
+  // wave solver: three fields, the source and the cell acceleration
+  float* d_wave[3] = {nullptr, nullptr, nullptr};
+  float* d_src = nullptr;
+  float* d_cacc = nullptr;
+  if (opt.wave) {
+    for (auto& f : d_wave) f = db.alloc<float>(G);
+    d_src = db.alloc<float>(G);
+    d_cacc = db.alloc<float>(3 * G);
+    if (!d_cacc) {
+      std::fprintf(stderr, "Error: device allocation failed\n");
+      return 1;
+    }
+    for (auto& f : d_wave) CHECK_HIP(hipMemset(f, 0, G * sizeof(float)));
+  }
   if (!d_cphi) {
     std::fprintf(stderr, "Error: device allocation failed\n");
     return 1;
@@ -307,9 +341,49 @@ int run(const Options& opt) {
     return true;
   };
 
-  // initial PHI and ACCEL (nbody.cu:1468-1481)
-  CHECK_K(ctx, khrr_accel(ctx, n, d_pos32, d_h32, n, d_pos32, d_m32, d_h32, d_acc, d_phi, gflags,
-                          nullptr));
+  // wave constants (nbody.cu:1134-1143) on the driver's grid
+  khrr_wave wave{};
+  wave.nx = nx;
+  wave.dx = opt.grid_dx;
+  wave.c = 4574.337022617616;
+  wave.dt = 0.5 * opt.grid_dx / (wave.c * std::sqrt(3.0));
+  wave.diss_base = 0.1;
+  wave.diss_extra = 0.01;
+  wave.sim_l = sim_l;
+  wave.bc_l = sim_l / 10.;
+  float* w_prev = d_wave[0];
+  float* w_curr = d_wave[1];
+  float* w_next = d_wave[2];
+  // calc_acceleration_by_grid (nbody.cu:183-285) at the FP64 positions x
+  auto mesh_force = [&](const double* x, int iters, float* acc_out) {
+    int rc = khrr_p2mesh_bin(ctx, &grid, n, x, d_mass, d_cell, d_idc, d_count, d_off, nullptr,
+                             nullptr, d_cmass);
+    if (!rc) rc = khrr_wave_source(ctx, &wave, d_cmass, d_src);
+    if (!rc) rc = khrr_wave_iterate(ctx, &wave, &w_prev, &w_curr, &w_next, d_src, iters);
+    if (!rc) rc = khrr_mesh_acc(ctx, nx, opt.grid_dx, w_curr, d_cacc, nullptr);
+    if (!rc) rc = khrr_mesh_to_particles(ctx, n, G, d_cell, d_cacc, w_curr, acc_out, d_phi);
+    return rc;
+  };
+  auto write_wave_field = [&](int frame) {
+    std::vector<float> f(G);
+    if (hipMemcpy(f.data(), w_curr, G * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+      return false;
+    write_cell_phi(out, frame, nx, sim_l, opt.grid_dx, std::vector<double>(f.begin(), f.end()));
+    return true;
+  };
+  const int step_iters = (int)(dtgrav / wave.dt);  // nbody.cu:1556
+
+  // initial PHI and ACCEL (nbody.cu:1449-1481)
+  if (opt.wave) {
+    CHECK_K(ctx, mesh_force(d_pos, opt.setup_iters, d_acc));
+    if (!write_wave_field(0)) {
+      std::fprintf(stderr, "Error: wave field copy failed\n");
+      return 1;
+    }
+  } else {
+    CHECK_K(ctx, khrr_accel(ctx, n, d_pos32, d_h32, n, d_pos32, d_m32, d_h32, d_acc, d_phi, gflags,
+                            nullptr));
+  }
   if (it == 1) {
     Diag d0;
     if (!diag(d0)) {
@@ -322,16 +396,20 @@ int run(const Options& opt) {
   auto wall0 = std::chrono::steady_clock::now();
   long steps = 0;
   int it_grav = 0;
-  double pair_evals = (double)n * n;  // initial force
+  double pair_evals = opt.wave ? 0.0 : (double)n * n;  // initial force
   do {
     // predictor (tn + dtgrav), gravity at x*, corrector (nbody.cu:1525-1582)
     CHECK_K(ctx, khrr_integrate(ctx, n, 0, dtgrav, tgrav - dtgrav, &ext, d_acc, d_post, d_velt,
                                 d_pos, d_vel, nullptr, d_pos32));
-    CHECK_K(ctx, khrr_accel(ctx, n, d_pos32, d_h32, n, d_pos32, d_m32, d_h32, d_acct, nullptr,
-                            step_flags, nullptr));
+    if (opt.wave) {
+      CHECK_K(ctx, mesh_force(d_post, step_iters, d_acct));
+    } else {
+      CHECK_K(ctx, khrr_accel(ctx, n, d_pos32, d_h32, n, d_pos32, d_m32, d_h32, d_acct, nullptr,
+                              step_flags, nullptr));
+      pair_evals += (double)n * n;
+    }
     CHECK_K(ctx, khrr_integrate(ctx, n, 1, dtgrav, tgrav, &ext, d_acct, d_pos, d_vel, d_post,
                                 d_velt, d_acc, nullptr));
-    pair_evals += (double)n * n;
     ++steps;
     ++itg;
     t = tgrav;
@@ -339,16 +417,24 @@ int run(const Options& opt) {
     ++it_grav;
     if (it_grav >= is_grav) {
       // save block (nbody.cu:1597-1707)
-      CHECK_K(ctx, khrr_accel(ctx, n, d_pos32, d_h32, n, d_pos32, d_m32, d_h32, nullptr, d_phi,
-                              gflags, nullptr));
-      pair_evals += (double)n * n;
-      CHECK_K(ctx, khrr_p2mesh_bin(ctx, &grid, n, d_pos, d_mass, d_cell, d_idc, d_count, d_off,
-                                   nullptr, nullptr, d_cmass));
-      CHECK_K(ctx, khrr_p2mesh_cell_phi(ctx, n, G, d_cell, d_idc, d_count, d_off, d_phi, d_cphi,
-                                        opt.clear_cell_phi ? 0 : 1));
-      std::vector<double> cphi(G);
-      CHECK_HIP(hipMemcpy(cphi.data(), d_cphi, G * sizeof(double), hipMemcpyDeviceToHost));
-      write_cell_phi(out, itt * it, nx, sim_l, opt.grid_dx, cphi);
+      if (opt.wave) {
+        // phi is the mesh potential at the bodies; the wave field is written
+        if (!write_wave_field(itt * it)) {
+          std::fprintf(stderr, "Error: wave field copy failed\n");
+          return 1;
+        }
+      } else {
+        CHECK_K(ctx, khrr_accel(ctx, n, d_pos32, d_h32, n, d_pos32, d_m32, d_h32, nullptr, d_phi,
+                                gflags, nullptr));
+        pair_evals += (double)n * n;
+        CHECK_K(ctx, khrr_p2mesh_bin(ctx, &grid, n, d_pos, d_mass, d_cell, d_idc, d_count, d_off,
+                                     nullptr, nullptr, d_cmass));
+        CHECK_K(ctx, khrr_p2mesh_cell_phi(ctx, n, G, d_cell, d_idc, d_count, d_off, d_phi, d_cphi,
+                                          opt.clear_cell_phi ? 0 : 1));
+        std::vector<double> cphi(G);
+        CHECK_HIP(hipMemcpy(cphi.data(), d_cphi, G * sizeof(double), hipMemcpyDeviceToHost));
+        write_cell_phi(out, itt * it, nx, sim_l, opt.grid_dx, cphi);
+      }
       if (n > 1) {
         double p1[3], v1[3], a1[3];
         float a1f[3], phi1 = 0.f;

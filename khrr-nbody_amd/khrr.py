@@ -115,6 +115,8 @@ _SIGS = {
     "khrr_p2mesh_bin": (_I, [_P, ctypes.POINTER(Grid), _I, _P, _P, _P, _P, _P, _P, _P, _P,
                              _P]),
     "khrr_p2mesh_cell_phi": (_I, [_P, _I, ctypes.c_longlong, _P, _P, _P, _P, _P, _P, _I]),
+    "khrr_mesh_acc": (_I, [_P, _I, _D, _P, _P, _P]),
+    "khrr_mesh_to_particles": (_I, [_P, _I, ctypes.c_longlong, _P, _P, _P, _P, _P]),
     "khrr_server_create": (_I, [_P, ctypes.c_size_t, _I, ctypes.POINTER(_P)]),
     "khrr_server_destroy": (None, [_P]),
     "khrr_server_set_round_pairs": (_I, [_P, _D]),
@@ -384,6 +386,28 @@ class Context:
             self._h, ctypes.byref(wave), ctypes.byref(p), ctypes.byref(c), ctypes.byref(x),
             _ptr(src, f32, (G,), "src"), iters))
         return bufs[p.value], bufs[c.value], bufs[x.value]
+
+    # -- mesh force of the wave solver ------------------------------------
+    def mesh_acc(self, nx: int, dx: float, cell_phi, cell_acc, cell_acc_abs=None):
+        """acc_field: cell_acc (3, nx^3) float32 = -grad(cell_phi) (nx^3 float32)."""
+        import torch
+        f32 = torch.float32
+        G = nx ** 3
+        self._check(self._lib.khrr_mesh_acc(
+            self._h, int(nx), float(dx), _ptr(cell_phi, f32, (G,), "cell_phi"),
+            _ptr(cell_acc, f32, (3, G), "cell_acc"),
+            _ptr(cell_acc_abs, f32, (G,), "cell_acc_abs")))
+
+    def mesh_to_particles(self, cell_id, cell_acc=None, cell_phi=None, acc=None, phi=None):
+        """acc[:, i] = cell_acc[:, cell_id[i]], phi[i] = cell_phi[cell_id[i]]."""
+        import torch
+        n = cell_id.shape[0]
+        G = (cell_phi if cell_phi is not None else cell_acc).shape[-1]
+        f32 = torch.float32
+        self._check(self._lib.khrr_mesh_to_particles(
+            self._h, n, G, _ptr(cell_id, torch.int32, (n,), "cell_id"),
+            _ptr(cell_acc, f32, (3, G), "cell_acc"), _ptr(cell_phi, f32, (G,), "cell_phi"),
+            _ptr(acc, f32, (3, n), "acc"), _ptr(phi, f32, (n,), "phi")))
 
 
 def reference_wave(nx: int, dx: float) -> Wave:

@@ -410,3 +410,48 @@ void orc_p2mesh_cell_phi(int n, long long G, const int* cell_id, const int* coun
     cell_phi[c] += phi[i] / count[c];
   }
 }
+
+/* ---------------------------------------------------------------------- */
+/* acc_field p2mesh-kernel.cuh:169-224 with every cell at its own (ix, iy,
+ * iz) (the 3-D stencil the kernel's indexing describes): per axis a
+ * one-sided difference on the low face, on the high face, else half the
+ * central difference; then times -1/dx.  acc: vec3 planes of nx^3;
+ * acc_abs (may be NULL): sqrt(ax*ax + ay*ay + az*az) (norm3, common.cuh:94). */
+static double orc_diff1(int i, int nx, double minus, double c, double plus) {
+  if (i == 0) return plus - c;
+  if (i == nx - 1) return c - minus;
+  return 0.5 * (plus - minus);
+}
+
+void orc_mesh_acc(int nx, double dx, const double* phi, double* acc, double* acc_abs) {
+  const size_t N = (size_t)nx, G = N * N * N;
+  const double coef = -1. / dx;
+  for (int iz = 0; iz < nx; ++iz)
+    for (int iy = 0; iy < nx; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        const size_t at = (size_t)ix + (size_t)iy * N + (size_t)iz * N * N;
+        const double c = phi[at];
+        double gx = orc_diff1(ix, nx, ix > 0 ? phi[at - 1] : 0.0, c, ix + 1 < nx ? phi[at + 1] : 0.0);
+        double gy = orc_diff1(iy, nx, iy > 0 ? phi[at - N] : 0.0, c, iy + 1 < nx ? phi[at + N] : 0.0);
+        double gz = orc_diff1(iz, nx, iz > 0 ? phi[at - N * N] : 0.0, c,
+                              iz + 1 < nx ? phi[at + N * N] : 0.0);
+        gx *= coef;
+        gy *= coef;
+        gz *= coef;
+        acc[at] = gx;
+        acc[G + at] = gy;
+        acc[2 * G + at] = gz;
+        if (acc_abs) acc_abs[at] = sqrt(gx * gx + gy * gy + gz * gz);
+      }
+}
+
+/* apply_acceleration_to_particles / apply_cell_to_particles
+ * p2mesh-kernel.cuh:226-252: value of the particle's cell. */
+void orc_mesh_to_particles(int n, long long G, const int* cell_id, const double* cell_acc,
+                           const double* cell_phi, double* acc, double* phi) {
+  for (int i = 0; i < n; ++i) {
+    const long long c = cell_id[i];
+    for (int k = 0; k < 3; ++k) acc[(size_t)k * n + i] = cell_acc[k * G + c];
+    phi[i] = cell_phi[c];
+  }
+}

@@ -31,6 +31,10 @@
  *                      faces = 0, independent numpy restatement, single-cell
  *                      impulse response by hand.
  *   orc_diag           closed forms on hand-made states, numpy sums.
+ *   orc_mesh_acc       linear fields (exact gradient everywhere), quadratic
+ *                      fields (exact inside, closed-form face error),
+ *                      numpy.gradient with first-order edges.
+ *   orc_mesh_to_particles  numpy fancy indexing.
  */
 #ifndef KHRR_ORACLE_H
 #define KHRR_ORACLE_H
@@ -101,6 +105,12 @@ void orc_wave_iterate(int nx, double dx, double c, double dt, double diss_base,
                       double diss_extra, double sim_l, double bc_l, const double* prev0,
                       const double* curr0, const double* mass, int iters, double* phi_out,
                       double* prev_out);
+
+/* acc_field p2mesh-kernel.cuh:169-224 on the full 3-D grid (FP64). */
+void orc_mesh_acc(int nx, double dx, const double* phi, double* acc, double* acc_abs);
+/* apply_acceleration_to_particles / apply_cell_to_particles :226-252. */
+void orc_mesh_to_particles(int n, long long G, const int* cell_id, const double* cell_acc,
+                           const double* cell_phi, double* acc, double* phi);
 
 /* Threads OpenMP will use (for the baseline's core count). */
 int orc_num_threads(void);

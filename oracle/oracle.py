@@ -49,6 +49,8 @@ def lib():
         L.orc_sat_trunc.restype = _I
         L.orc_wave_step.argtypes = [_I] + [_D] * 7 + [_P] * 4
         L.orc_wave_iterate.argtypes = [_I] + [_D] * 7 + [_P, _P, _P, _I, _P, _P]
+        L.orc_mesh_acc.argtypes = [_I, _D, _P, _P, _P]
+        L.orc_mesh_to_particles.argtypes = [_I, ctypes.c_longlong, _P, _P, _P, _P, _P]
         L.orc_num_threads.restype = _I
         _lib = L
     return _lib
@@ -203,3 +205,26 @@ def reference_wave(nx, dx):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def mesh_acc(nx, dx, phi, want_abs=True):
+    """acc_field on the full 3-D grid: returns (acc (3, nx^3), |acc| or None)."""
+    phi = _f64(phi)
+    G = nx ** 3
+    acc = np.zeros((3, G))
+    ab = np.zeros(G) if want_abs else None
+    lib().orc_mesh_acc(nx, float(dx), _p(phi), _p(acc), _p(ab))
+    return acc, ab
+
+
+def mesh_to_particles(cell_id, cell_acc, cell_phi):
+    """Per-particle cell values: returns (acc (3, n), phi (n,))."""
+    cell_id = np.ascontiguousarray(cell_id, dtype=np.int32)
+    cell_acc = _f64(cell_acc)
+    cell_phi = _f64(cell_phi)
+    n = cell_id.shape[0]
+    acc = np.zeros((3, n))
+    phi = np.zeros(n)
+    lib().orc_mesh_to_particles(n, cell_phi.shape[0], _p(cell_id), _p(cell_acc), _p(cell_phi),
+                                _p(acc), _p(phi))
+    return acc, phi

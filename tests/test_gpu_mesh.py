@@ -1,0 +1,99 @@
+"""GPU mesh force (khrr_mesh_acc, khrr_mesh_to_particles) against the FP64
+oracle: the FP32 field is widened, differenced and scaled in FP64 and
+rounded once, so the acceleration field matches the oracle rounded to FP32
+bit for bit; the whole wave-solver chain (bin -> source -> iterate -> mesh
+acceleration -> particles) matches the FP64 chain within the stencil's
+1e-5 tolerance."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from helpers import to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("nx", [2, 37, 64, 70])
+def test_field_bit_exact(ctx, nx):
+    import torch
+    G = nx ** 3
+    dx = 4.0 / nx
+    phi = np.random.default_rng(nx).normal(size=G).astype(np.float32) * 25.0
+    acc = torch.zeros((3, G), device="cuda:0")
+    ab = torch.zeros(G, device="cuda:0")
+    ctx.mesh_acc(nx, dx, to_dev(phi, torch.float32), acc, ab)
+    torch.cuda.synchronize()
+    ref, ref_ab = orc.mesh_acc(nx, dx, phi.astype(np.float64))
+    np.testing.assert_array_equal(acc.cpu().numpy(), ref.astype(np.float32))
+    np.testing.assert_allclose(ab.cpu().numpy(), ref_ab.astype(np.float32), rtol=1.2e-7)
+
+
+def test_gather_and_bad_ids(ctx):
+    import torch
+    rng = np.random.default_rng(9)
+    G = 1000
+    cell_acc = rng.normal(size=(3, G)).astype(np.float32)
+    cell_phi = rng.normal(size=G).astype(np.float32)
+    cid = rng.integers(0, G, 4096).astype(np.int32)
+    cid[7] = -1
+    cid[8] = G
+    acc = torch.zeros((3, cid.size), device="cuda:0")
+    phi = torch.zeros(cid.size, device="cuda:0")
+    ctx.mesh_to_particles(to_dev(cid, torch.int32), to_dev(cell_acc, torch.float32),
+                          to_dev(cell_phi, torch.float32), acc, phi)
+    torch.cuda.synchronize()
+    a, p = acc.cpu().numpy(), phi.cpu().numpy()
+    ok = (cid >= 0) & (cid < G)
+    np.testing.assert_array_equal(a[:, ok], cell_acc[:, cid[ok]])
+    np.testing.assert_array_equal(p[ok], cell_phi[cid[ok]])
+    assert np.isnan(a[:, ~ok]).all() and np.isnan(p[~ok]).all()
+
+
+def test_rejects_bad_arguments(ctx):
+    import khrr
+    import torch
+    f = torch.zeros(8, device="cuda:0")
+    with pytest.raises(khrr.KhrrError):
+        ctx.mesh_acc(1, 1.0, torch.zeros(1, device="cuda:0"), torch.zeros((3, 1), device="cuda:0"))
+    with pytest.raises(khrr.KhrrError):
+        ctx.mesh_acc(2, 0.0, f, torch.zeros((3, 8), device="cuda:0"))
+
+
+def test_wave_solver_chain(ctx):
+    # calc_acceleration_by_grid (nbody.cu:183-285): bin the bodies, iterate
+    # the damped wave equation, take -grad(phi) and read it at the bodies
+    import khrr
+    import torch
+    from helpers import galaxy_system
+    pos, _, mass, _ = galaxy_system(3000, 6000, seed=12, eps_s=0.004, eps_d=0.004)
+    n = pos.shape[1]
+    nx, iters = 40, 120
+    dx = 4.0 / nx
+    grid = khrr.cube_grid(nx, -2.0, 2.0)
+    G = nx ** 3
+    i32 = dict(device="cuda:0", dtype=torch.int32)
+    cell_id = torch.zeros(n, **i32)
+    count = torch.zeros(G, **i32)
+    cm = torch.zeros(G, device="cuda:0", dtype=torch.float64)
+    ctx.p2mesh_bin(grid, to_dev(pos, torch.float64), to_dev(mass, torch.float64),
+                   cell_id=cell_id, count=count, cell_mass=cm)
+    w = khrr.reference_wave(nx, dx)
+    src = torch.zeros(G, device="cuda:0")
+    ctx.wave_source(w, cm, src)
+    b = [torch.zeros(G, device="cuda:0") for _ in range(3)]
+    _, cur, _ = ctx.wave_iterate(w, b[0], b[1], b[2], src, iters)
+    cacc = torch.zeros((3, G), device="cuda:0")
+    ctx.mesh_acc(nx, dx, cur, cacc)
+    acc = torch.zeros((3, n), device="cuda:0")
+    phi = torch.zeros(n, device="cuda:0")
+    ctx.mesh_to_particles(cell_id, cacc, cur, acc, phi)
+    torch.cuda.synchronize()
+    wd = orc.reference_wave(nx, dx)
+    ref_phi, _ = orc.wave_iterate(wd, np.zeros(G), np.zeros(G), cm.cpu().numpy(), iters)
+    ref_cacc, _ = orc.mesh_acc(nx, dx, ref_phi, want_abs=False)
+    ref_acc, ref_p = orc.mesh_to_particles(cell_id.cpu().numpy(), ref_cacc, ref_phi)
+    a = acc.cpu().numpy().astype(np.float64)
+    scale = np.max(np.abs(ref_acc))
+    assert scale > 0
+    assert np.max(np.abs(a - ref_acc)) / scale <= 1e-4
+    assert np.max(np.abs(phi.cpu().numpy() - ref_p)) / np.max(np.abs(ref_p)) <= 1e-5

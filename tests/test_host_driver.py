@@ -102,3 +102,22 @@ def test_small_galaxy_run_outputs(tmp_path):
     np.testing.assert_allclose(E[:, 2], E_ref, rtol=1e-5)
     np.testing.assert_allclose(L0[2], d[0, 7], rtol=1e-9)
     np.testing.assert_array_equal(E[:, 0], np.arange(11))
+
+
+@pytest.mark.gpu
+def test_wave_solver_run(tmp_path):
+    # the reference's NBodySolver::Wave branch: mesh force from the damped
+    # wave field (setup iterations, then dtgrav/dt_wave per step)
+    base = _layout(str(tmp_path), [dict(n_s=1024, n_dm=2048, seed=6)], 0.02, 0.002)
+    r = subprocess.run([BIN, "--data", base, "--grid", "32", "0.125", "--solver", "wave",
+                        "--setup-iters", "600", "--max-steps", "4", "--json"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = os.path.join(base, "out")
+    E = np.genfromtxt(os.path.join(out, "E(t).csv"), delimiter=",", skip_header=1)
+    assert E.shape == (5, 7) and np.isfinite(E).all()
+    # Ep = sum m phi with the mesh potential: bound system, negative
+    assert (E[:, 6] < 0).all()
+    for frame in ("    0", "    1", "    4"):
+        assert os.path.exists(os.path.join(out, "phi", f"phi_{frame}.csv"))
+    assert os.path.getsize(os.path.join(base, "bin", "LIE_0.bin")) == 72
