@@ -61,6 +61,13 @@ extern "C" {
  * sums are folded into a (hi, lo) pair so the FP32 result carries the
  * rounding of the pair terms only, not of an N-term running sum. */
 #define KHRR_COMPENSATED 2u
+/* khrr_accel_batched only: the call's launch sequence is captured into a
+ * HIP graph, cached per (composition, pointers, flags, launch shape), and
+ * later identical calls replay it with one hipGraphLaunch.  The first call
+ * of a key runs eagerly and captures on a private stream; results are the
+ * eager results bit for bit.  Up to 4 graphs are kept per context (least
+ * recently used evicted). */
+#define KHRR_GRAPH 4u
 
 typedef struct khrr_ctx khrr_ctx;
 

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "khrr_internal.h"
 #include "khrr_pair.h"
@@ -239,6 +240,9 @@ struct WorkList {
   int n_sym_sys = 0;
   int max_sym_n = 0;
   long long sym_floats = 0;
+  const int2* tiles = nullptr;
+  int n_tiles = 0;
+  size_t dev_bytes = 0;  // size of the device block the pointers live in
 };
 
 // Byte offsets of the tables in the device block (and the pinned copy).
@@ -260,10 +264,10 @@ struct Layout {
 // Builds (or reuses) the device work list for this composition: one-sided
 // items longest system first, the flag ranges of every system (j-tiles, or
 // superblocks on the pair-symmetric path), the cut systems' reduction
-// list, the pair-symmetric items and systems; then the flags (recomputed on
-// every call: masses and softening may change).
-int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* host_count,
-                  const float* h, const float* m, int ipl, int j_tile, WorkList* wl) {
+// list, the pair-symmetric items and systems.  The flags themselves are
+// recomputed by every launch (masses and softening may change).
+int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* host_count, int ipl,
+                  int j_tile, WorkList* wl) {
   std::vector<int> key(2 * (size_t)n_sys + 1);
   for (int s = 0; s < n_sys; ++s) {
     key[2 * s] = host_begin[s];
@@ -386,10 +390,29 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
   wl->n_sym_sys = ctx->batch_sym_sys;
   wl->max_sym_n = ctx->batch_sym_max;
   wl->sym_floats = ctx->batch_sym_floats;
-  if (ctx->batch_tiles > 0)
-    hipLaunchKernelGGL(tile_flags_kernel, dim3(khrr::ceil_div(ctx->batch_tiles, 4)), dim3(256), 0,
-                       ctx->stream, ctx->batch_tiles, tiles, h, m, wl->tile_flags);
-  return khrr::check_launch(ctx, "tile_flags_kernel");
+  wl->tiles = tiles;
+  wl->n_tiles = ctx->batch_tiles;
+  wl->dev_bytes = L.dev_bytes;
+  return 0;
+}
+
+// The same work list with every pointer moved from block `from` to `to`.
+WorkList rebase(const WorkList& w, const void* from, void* to) {
+  auto mv = [&](auto* p) {
+    using T = std::remove_reference_t<decltype(*p)>;
+    return p ? reinterpret_cast<T*>(static_cast<char*>(to) +
+                                    (reinterpret_cast<const char*>(p) -
+                                     static_cast<const char*>(from)))
+             : p;
+  };
+  WorkList r = w;
+  r.items = mv(w.items);
+  r.tile_flags = mv(w.tile_flags);
+  r.red = mv(w.red);
+  r.sym_items = mv(w.sym_items);
+  r.sym_sys = mv(w.sym_sys);
+  r.tiles = mv(w.tiles);
+  return r;
 }
 
 // Resident blocks per CU of the persistent kernel for this shape (cached).
@@ -407,6 +430,49 @@ int blocks_per_cu(int ipl, bool a, bool p, bool c, int wpb) {
     v = nb;
   }
   return v;
+}
+
+// The launches of one batched evaluation over a prepared work list: tile
+// flags, the pair-symmetric items and their slot reduction, the persistent
+// one-sided grid and the piece reduction.  No host synchronisation and no
+// allocation, so the sequence can be captured into a graph.
+int launch_work(khrr_ctx* ctx, const WorkList& wl, int ld, const float* pos, const float* h,
+                const float* m, float* acc, float* phi, unsigned flags, const khrr_tune& t,
+                float* part, float* sym_part) {
+  using namespace khrr;
+  const bool comp = (flags & KHRR_COMPENSATED) != 0;
+  const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
+  if (wl.n_tiles > 0)
+    hipLaunchKernelGGL(tile_flags_kernel, dim3(ceil_div(wl.n_tiles, 4)), dim3(256), 0, ctx->stream,
+                       wl.n_tiles, wl.tiles, h, m, wl.tile_flags);
+  KHRR_TRY(check_launch(ctx, "tile_flags_kernel"));
+  // pair-symmetric systems first (one workgroup per (A, B) item) ...
+  KHRR_TRY(sym_items_launch(ctx, wl.n_sym_big, wl.n_sym_items - wl.n_sym_big, wl.sym_items,
+                            wl.n_sym_sys, wl.sym_sys, wl.max_sym_n, ld, pos, h, m, wl.tile_flags,
+                            sym_part, acc, phi, flags));
+  if (wl.n_items == 0) return 0;
+  // ... then the persistent one-sided grid: as many blocks as stay resident,
+  // never more waves than items
+  BatchLauncher fn = pick_batched(t.i_per_lane, acc != nullptr, phi != nullptr, comp);
+  const int wpb = t.waves_per_block;
+  const long long resident =
+      (long long)blocks_per_cu(t.i_per_lane, acc != nullptr, phi != nullptr, comp, wpb) *
+      ctx->cu_count;
+  const long long blocks = std::min<long long>(resident, ceil_div(wl.n_items, wpb));
+  int* counter = ctx->dev_words;
+  hipLaunchKernelGGL(set_word, dim3(1), dim3(1), 0, ctx->stream, counter, 0);
+  fn(dim3((unsigned)blocks), dim3(64 * wpb), ctx->stream, wl.n_items, wl.items, wl.tile_flags, ld,
+     pos, h, m, t.j_tile, counter, acc, phi, part, accumulate);
+  if (wl.n_red > 0) {
+    const dim3 grid(ceil_div(wl.max_red_count, 256), wl.n_red);
+    if (comp)
+      hipLaunchKernelGGL(batched_reduce<true>, grid, dim3(256), 0, ctx->stream, wl.red, ld,
+                         (int)(acc != nullptr), (int)(phi != nullptr), part, acc, phi, accumulate);
+    else
+      hipLaunchKernelGGL(batched_reduce<false>, grid, dim3(256), 0, ctx->stream, wl.red, ld,
+                         (int)(acc != nullptr), (int)(phi != nullptr), part, acc, phi, accumulate);
+  }
+  return check_launch(ctx, "batched_kernel");
 }
 
 }  // namespace
@@ -431,10 +497,10 @@ int batched_segments(khrr_ctx* ctx, int n_sys, const int* host_begin, const int*
   for (int s = 0; s < n_sys; ++s) total += host_count[s];
   if (total == 0 || n_sys == 0) return 0;
   WorkList wl;
-  KHRR_TRY(prepare_items(ctx, n_sys, host_begin, host_count, h, m, t.i_per_lane, t.j_tile, &wl));
+  KHRR_TRY(prepare_items(ctx, n_sys, host_begin, host_count, t.i_per_lane, t.j_tile, &wl));
   const bool comp = (flags & KHRR_COMPENSATED) != 0;
-  BatchLauncher fn = pick_batched(t.i_per_lane, acc != nullptr, phi != nullptr, comp);
-  if (!fn) return fail(ctx, KHRR_EINVAL, "unsupported i_per_lane %d", t.i_per_lane);
+  if (!pick_batched(t.i_per_lane, acc != nullptr, phi != nullptr, comp))
+    return fail(ctx, KHRR_EINVAL, "unsupported i_per_lane %d", t.i_per_lane);
   float* part = nullptr;
   float* sym_part = nullptr;
   if (wl.n_red > 0 || wl.n_sym_items > 0) {
@@ -446,34 +512,81 @@ int batched_segments(khrr_ctx* ctx, int n_sys, const int* host_begin, const int*
     part = cv.take<float>((size_t)(comp ? 8 : 4) * wl.part_units);
     sym_part = cv.take<float>((size_t)wl.sym_floats);
   }
-  const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
-  Timed tm(ctx, KHRR_K_GRAVITY);
-  // pair-symmetric systems first (one workgroup per (A, B) item) ...
-  KHRR_TRY(sym_items_launch(ctx, wl.n_sym_big, wl.n_sym_items - wl.n_sym_big, wl.sym_items,
-                            wl.n_sym_sys, wl.sym_sys, wl.max_sym_n, ld, pos, h, m, wl.tile_flags,
-                            sym_part, acc, phi, flags));
-  if (wl.n_items == 0) return 0;
-  // ... then the persistent one-sided grid: as many blocks as stay resident,
-  // never more waves than items
-  const int wpb = t.waves_per_block;
-  const long long resident =
-      (long long)blocks_per_cu(t.i_per_lane, acc != nullptr, phi != nullptr, comp, wpb) *
-      ctx->cu_count;
-  const long long blocks = std::min<long long>(resident, ceil_div(wl.n_items, wpb));
-  int* counter = ctx->dev_words;
-  hipLaunchKernelGGL(set_word, dim3(1), dim3(1), 0, ctx->stream, counter, 0);
-  fn(dim3((unsigned)blocks), dim3(64 * wpb), ctx->stream, wl.n_items, wl.items, wl.tile_flags, ld,
-     pos, h, m, t.j_tile, counter, acc, phi, part, accumulate);
-  if (wl.n_red > 0) {
-    const dim3 grid(ceil_div(wl.max_red_count, 256), wl.n_red);
-    if (comp)
-      hipLaunchKernelGGL(batched_reduce<true>, grid, dim3(256), 0, ctx->stream, wl.red, ld,
-                         (int)(acc != nullptr), (int)(phi != nullptr), part, acc, phi, accumulate);
-    else
-      hipLaunchKernelGGL(batched_reduce<false>, grid, dim3(256), 0, ctx->stream, wl.red, ld,
-                         (int)(acc != nullptr), (int)(phi != nullptr), part, acc, phi, accumulate);
+  const unsigned kf = flags & ~KHRR_GRAPH;
+  if (!(flags & KHRR_GRAPH)) {
+    Timed tm(ctx, KHRR_K_GRAVITY);
+    return launch_work(ctx, wl, ld, pos, h, m, acc, phi, kf, t, part, sym_part);
   }
-  return check_launch(ctx, "batched_kernel");
+
+  // Graph path: the key is everything the captured launches depend on.
+  std::vector<long long> key(ctx->batch_off.begin(), ctx->batch_off.end());
+  for (long long v : {(long long)t.i_per_lane, (long long)t.waves_per_block, (long long)ld,
+                      (long long)kf, (long long)(intptr_t)pos, (long long)(intptr_t)h,
+                      (long long)(intptr_t)m, (long long)(intptr_t)acc, (long long)(intptr_t)phi,
+                      (long long)(intptr_t)ctx->scratch, (long long)ctx->scratch_bytes})
+    key.push_back(v);
+  for (auto& g : ctx->graphs) {
+    if (g.key != key) continue;
+    g.last_use = ++ctx->graph_clock;
+    Timed tm(ctx, KHRR_K_GRAVITY);
+    if (g.failed) return launch_work(ctx, wl, ld, pos, h, m, acc, phi, kf, t, part, sym_part);
+    KHRR_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
+    return 0;
+  }
+  // First call of this key: run it eagerly, then capture a replayable copy.
+  {
+    Timed tm(ctx, KHRR_K_GRAVITY);
+    KHRR_TRY(launch_work(ctx, wl, ld, pos, h, m, acc, phi, kf, t, part, sym_part));
+  }
+  if (ctx->graphs.size() >= 4) {
+    auto lru = std::min_element(ctx->graphs.begin(), ctx->graphs.end(),
+                                [](const GraphEntry& a, const GraphEntry& b) {
+                                  return a.last_use < b.last_use;
+                                });
+    KHRR_HIP(ctx, hipStreamSynchronize(ctx->stream));  // a replay may still run
+    if (lru->exec) (void)hipGraphExecDestroy(lru->exec);
+    if (lru->graph) (void)hipGraphDestroy(lru->graph);
+    if (lru->dev) (void)hipFree(lru->dev);
+    ctx->graphs.erase(lru);
+  }
+  GraphEntry g;
+  g.key = key;
+  g.last_use = ++ctx->graph_clock;
+  bool ok = hipMalloc(&g.dev, wl.dev_bytes) == hipSuccess;
+  if (ok && !ctx->capture_stream)
+    ok = hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking) == hipSuccess;
+  if (ok)  // the work list as it is now, ordered after the eager run's use of it
+    ok = hipMemcpyAsync(g.dev, ctx->batch_dev, wl.dev_bytes, hipMemcpyDeviceToDevice,
+                        ctx->stream) == hipSuccess &&
+         hipStreamSynchronize(ctx->stream) == hipSuccess;
+  if (ok) {
+    const WorkList wg = rebase(wl, ctx->batch_dev, g.dev);
+    hipStream_t saved = ctx->stream;
+    const bool timing = ctx->timing;
+    ctx->stream = ctx->capture_stream;
+    ctx->timing = false;
+    ok = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    const int rc = ok ? launch_work(ctx, wg, ld, pos, h, m, acc, phi, kf, t, part, sym_part) : 0;
+    hipGraph_t graph = nullptr;
+    if (ok) ok = hipStreamEndCapture(ctx->stream, &graph) == hipSuccess && rc == 0 && graph;
+    ctx->stream = saved;
+    ctx->timing = timing;
+    g.graph = graph;
+    if (ok) ok = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  }
+  if (!ok) {  // keep the eager path for this key
+    (void)hipGetLastError();
+    ctx->err.clear();
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    if (g.dev) (void)hipFree(g.dev);
+    g.exec = nullptr;
+    g.graph = nullptr;
+    g.dev = nullptr;
+    g.failed = true;
+  }
+  ctx->graphs.push_back(std::move(g));
+  return 0;
 }
 
 }  // namespace khrr
@@ -549,7 +662,7 @@ int khrr_accel_batched(khrr_ctx* ctx, int n_sys, const int* sys_off, const int* 
                        unsigned flags, const khrr_tune* tune) {
   KHRR_TRY(khrr::enter(ctx));
   KHRR_TRY(check_composition(ctx, n_sys, host_sys_off));
-  if (flags & ~(KHRR_ACCUMULATE | KHRR_COMPENSATED))
+  if (flags & ~(KHRR_ACCUMULATE | KHRR_COMPENSATED | KHRR_GRAPH))
     return khrr::fail(ctx, KHRR_EINVAL, "unknown flags 0x%x", flags);
   if (!acc && !phi) return khrr::fail(ctx, KHRR_EINVAL, "acc and phi are both NULL");
   if (n_sys > 0 && host_sys_off[n_sys] > 0 && (!pos || !h || !m))

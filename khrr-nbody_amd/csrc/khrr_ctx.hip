@@ -153,6 +153,12 @@ void khrr_ctx_destroy(khrr_ctx* ctx) {
     (void)hipEventDestroy(p.stop);
   }
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  for (auto& g : ctx->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    if (g.dev) (void)hipFree(g.dev);
+  }
+  if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->batch_dev) (void)hipFree(ctx->batch_dev);
   if (ctx->batch_pinned) (void)hipHostFree(ctx->batch_pinned);

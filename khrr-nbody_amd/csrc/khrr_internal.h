@@ -44,6 +44,18 @@ struct TimerPair {
   int cls;
 };
 
+// A captured batched gravity launch sequence (KHRR_GRAPH): its key (the
+// composition, every pointer and shape it was captured with, and the
+// scratch arena it uses) and a private copy of its device work list.
+struct GraphEntry {
+  std::vector<long long> key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void* dev = nullptr;  // work list copy (items, ranges, flags)
+  unsigned long long last_use = 0;
+  bool failed = false;  // capture not possible: this key runs eagerly
+};
+
 }  // namespace khrr
 
 struct khrr_ctx {
@@ -88,6 +100,10 @@ struct khrr_ctx {
   size_t batch_dev_bytes = 0;
   void* batch_pinned = nullptr;
   size_t batch_pinned_bytes = 0;
+  // Captured batched launches (KHRR_GRAPH), least recently used evicted.
+  std::vector<khrr::GraphEntry> graphs;
+  unsigned long long graph_clock = 0;
+  hipStream_t capture_stream = nullptr;  // private, non-blocking, default priority
 };
 
 namespace khrr {

@@ -13,7 +13,8 @@
 //   2. one step for the round's systems: every active system, or (with a
 //      round_pairs target) those with the least remaining work until the
 //      round holds that many pair-interactions; ONE predictor launch, ONE
-//      batched gravity launch and ONE corrector launch over all of them;
+//      batched gravity evaluation (replayed as a captured HIP graph while
+//      the round's composition repeats) and ONE corrector launch;
 //   3. systems that reached their step count get their potential, final
 //      diagnostics and the p2mesh mass grid, and release their memory.
 // A system's arithmetic never mixes with another's (segment-local j loops,
@@ -132,14 +133,17 @@ struct khrr_server {
 
 namespace {
 
-int gravity_over(khrr_server* s, const std::vector<Sim*>& set, float* acc, float* phi) {
+// One batched gravity evaluation over `set`; flags may ask for the graph
+// path (rounds repeat a composition until a system joins or leaves).
+int gravity_over(khrr_server* s, const std::vector<Sim*>& set, float* acc, float* phi,
+                 unsigned flags = 0u) {
   std::vector<int> begin, count;
   for (Sim* m : set) {
     begin.push_back(m->begin);
     count.push_back(m->n);
   }
   return khrr::batched_segments(s->ctx, (int)set.size(), begin.data(), count.data(), s->cap,
-                                s->pos32, s->h32, s->m32, acc, phi, 0u, s->tune);
+                                s->pos32, s->h32, s->m32, acc, phi, flags, s->tune);
 }
 
 int upload(khrr_server* s, Sim* m) {
@@ -384,7 +388,7 @@ int khrr_server_step(khrr_server* s, long long* done_ids, int max_done, int* n_d
                                  hipMemcpyHostToDevice, ctx->stream));
     KHRR_TRY(khrr::integrate_segments(ctx, n_seg, s->seg_dev, blocks, 0, s->acc, nullptr,
                                       s->pos32, s->cap));
-    KHRR_TRY(gravity_over(s, stepping, s->acct, nullptr));
+    KHRR_TRY(gravity_over(s, stepping, s->acct, nullptr, KHRR_GRAPH));
     KHRR_TRY(khrr::integrate_segments(ctx, n_seg, s->seg_dev, blocks, 1, s->acct, s->acc, nullptr,
                                       s->cap));
     for (Sim* m : stepping) {

@@ -31,6 +31,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "khrr.h")
 
 ACCUMULATE = 1
 COMPENSATED = 2
+GRAPH = 4
 
 K_GRAVITY, K_INTEGRATE, K_BIN, K_WAVE, K_DIAG = range(5)
 DIAG_NAMES = ("Ek", "Ep", "Px", "Py", "Pz", "Lx", "Ly", "Lz", "M", "Mx", "My", "Mz",
@@ -269,13 +270,16 @@ class Context:
             ctypes.byref(tune) if tune is not None else None))
 
     def accel_batched(self, sys_off_dev, sys_off_host, pos, h, m, acc=None, phi=None,
-                      accumulate=False, compensated=False, tune: Tune | None = None):
+                      accumulate=False, compensated=False, tune: Tune | None = None,
+                      graph=False):
+        """graph=True: capture/replay the launch sequence as a HIP graph."""
         import torch
         f32 = torch.float32
         n_sys = len(sys_off_host) - 1
         n = int(sys_off_host[-1])
         host = (ctypes.c_int * (n_sys + 1))(*[int(v) for v in sys_off_host])
-        flags = (ACCUMULATE if accumulate else 0) | (COMPENSATED if compensated else 0)
+        flags = ((ACCUMULATE if accumulate else 0) | (COMPENSATED if compensated else 0) |
+                 (GRAPH if graph else 0))
         self._check(self._lib.khrr_accel_batched(
             self._h, n_sys, _ptr(sys_off_dev, torch.int32, (n_sys + 1,), "sys_off"),
             ctypes.cast(host, _P), _ptr(pos, f32, (3, n), "pos"), _ptr(h, f32, (n,), "h"),

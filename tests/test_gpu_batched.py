@@ -163,3 +163,52 @@ def test_autotuners_return_valid_shapes(ctx):
         a, b = off[s], off[s + 1]
         pa, _ = _run_plain(ctx, pos[:, a:b], m[a:b], h[a:b])
         np.testing.assert_array_equal(acc[:, a:b], pa)
+
+
+def test_graph_replays_equal_eager(ctx):
+    # KHRR_GRAPH: the first call runs eagerly and captures; later calls with
+    # the same composition and pointers replay the graph.  Every replay,
+    # with eager work and synchronisations in between and with the inputs
+    # changed in place, must equal the eager result bit for bit.
+    import torch
+    sizes = [40000, 3072, 17000, 1024, 70000]
+    off, pos, m, h = _population(sizes, seed0=800)
+    n = pos.shape[1]
+    offd = to_dev(off, torch.int32)
+    P, H, M = to_dev(pos, torch.float32), to_dev(h, torch.float32), to_dev(m, torch.float32)
+    acc_g = torch.zeros((3, n), device="cuda:0")
+    phi_g = torch.zeros(n, device="cuda:0")
+    rng = np.random.default_rng(1)
+    for rep in range(5):
+        if rep >= 2:  # new positions in the same buffer
+            P.add_(to_dev(rng.normal(scale=1e-3, size=(3, n)), torch.float32))
+        acc_g.fill_(7.0)
+        ctx.accel_batched(offd, off.tolist(), P, H, M, acc=acc_g, phi=phi_g, graph=True)
+        torch.cuda.synchronize()
+        got_a, got_p = acc_g.cpu().numpy(), phi_g.cpu().numpy()
+        # eager work in between, then the eager reference on the same inputs
+        acc_e = torch.zeros((3, n), device="cuda:0")
+        phi_e = torch.zeros(n, device="cuda:0")
+        ctx.accel_batched(offd, off.tolist(), P, H, M, acc=acc_e, phi=phi_e)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(got_a, acc_e.cpu().numpy(), err_msg=f"replay {rep}")
+        np.testing.assert_array_equal(got_p, phi_e.cpu().numpy(), err_msg=f"replay {rep}")
+
+
+def test_graph_with_compositions_changing(ctx):
+    # more compositions than cached graphs: eviction and recapture stay exact
+    import torch
+    for k in range(6):
+        sizes = [1024 * (k + 1), 2048, 33000 + 1000 * (k % 2)]
+        off, pos, m, h = _population(sizes, seed0=900 + k)
+        n = pos.shape[1]
+        offd = to_dev(off, torch.int32)
+        P, H, M = to_dev(pos, torch.float32), to_dev(h, torch.float32), to_dev(m, torch.float32)
+        outs = []
+        for graph in (True, True, False):
+            acc = torch.zeros((3, n), device="cuda:0")
+            ctx.accel_batched(offd, off.tolist(), P, H, M, acc=acc, graph=graph)
+            torch.cuda.synchronize()
+            outs.append(acc.cpu().numpy())
+        np.testing.assert_array_equal(outs[0], outs[2])
+        np.testing.assert_array_equal(outs[1], outs[2])
