@@ -137,7 +137,7 @@ int khrr_accel(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int
  *  - n >= 32768 (when its partial slots fit, as for khrr_accel): the
  *    pair-symmetric path, one workgroup per (A, B) superblock item; the
  *    result is bit-identical to khrr_accel with i_per_lane 8, symmetric 1;
- *  - otherwise the j-range is cut into k = min(16, ceil(n / 16384)) pieces
+ *  - otherwise the j-range is cut into k = min(16, ceil(n / 2048)) pieces
  *    and (i-group, j-piece) items run longest system first in one
  *    persistent launch, pieces summed in piece order; the result is
  *    bit-identical to khrr_accel with j_split k, j_tile 256, symmetric 0.
@@ -325,11 +325,14 @@ typedef struct {
 } khrr_server_info;
 /* hbm_budget: device bytes the server may hold for simulation state: a
  * simulation of n bodies holds 152 B per body (FP32 gravity arrays and FP64
- * state) plus the batched gravity's partial sums for it (16 B per body per
- * superblock when n >= 32768 -- 2048 bodies below 131072, 4096 from there
- * on -- 16 B per body per j-piece when 16384 < n < 32768, else none).  The FP32 arena for budget / 152 bodies is
- * reserved at creation.  max_active: simultaneous simulations (>= 1). */
+ * state) plus the batched gravity's partial sums for it: 16 B per body per
+ * superblock when n >= 32768 (2048 bodies below 131072, 4096 from there
+ * on), else 16 B per body per j-piece when there are two or more.  The
+ * FP32 arena for budget / 152 bodies is reserved at creation.
+ * max_active: simultaneous simulations (>= 1). */
 int khrr_server_create(khrr_ctx* ctx, size_t hbm_budget, int max_active, khrr_server** out);
+/* Frees the server's device memory; call it before khrr_ctx_destroy of its
+ * context (the server works on the context's stream). */
 void khrr_server_destroy(khrr_server* srv);
 /* Round scheduling.  pairs == 0 (the default): every active simulation
  * takes one step per round.  pairs > 0: the active simulations are taken

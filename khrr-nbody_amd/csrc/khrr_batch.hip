@@ -37,7 +37,10 @@ struct RedItem {
 
 // Number of j-pieces of a system of n bodies: a rule of the system's own
 // size, so the summation order never depends on the rest of the batch.
-inline int pieces_of(int n) { return std::max(1, std::min(16, (n + 16383) / 16384)); }
+// Pieces of about 2048 j-bodies keep a wave's item short (256 i x 2048 j at
+// i_per_lane 4), so a round of a few small systems is not held up by a few
+// long waves.
+inline int pieces_of(int n) { return std::max(1, std::min(16, (n + 2047) / 2048)); }
 
 template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
 __global__ __launch_bounds__(256) void batched_kernel(

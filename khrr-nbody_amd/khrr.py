@@ -24,6 +24,7 @@ from __future__ import annotations
 import ctypes
 import os
 import re
+import weakref
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libkhrr.so")
@@ -201,6 +202,7 @@ class Context:
 
     def __init__(self, device: int = 0, stream=None):
         self._lib = lib()
+        self._servers = weakref.WeakSet()
         h = _P()
         rc = self._lib.khrr_ctx_create(device, ctypes.byref(h))
         if rc:
@@ -211,6 +213,9 @@ class Context:
             self.set_stream(stream)
 
     def close(self):
+        # servers hold the context: they go first
+        for srv in list(getattr(self, "_servers", ())):
+            srv.close()
         if getattr(self, "_h", None):
             self._lib.khrr_ctx_destroy(self._h)
             self._h = None
@@ -444,6 +449,7 @@ class Server:
         ctx._check(self._lib.khrr_server_create(ctx._h, int(budget_bytes), int(max_active),
                                                 ctypes.byref(h)))
         self._h = h
+        ctx._servers.add(self)
         if round_pairs:
             self.set_round_pairs(round_pairs)
 
@@ -456,6 +462,7 @@ class Server:
         if getattr(self, "_h", None):
             self._lib.khrr_server_destroy(self._h)
             self._h = None
+            self._ctx._servers.discard(self)
 
     def __del__(self):
         try:

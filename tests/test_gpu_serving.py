@@ -88,24 +88,39 @@ def test_energy_follows_oracle(ctx):
 
 def test_admission_respects_budget(ctx):
     import khrr
-    # 152 B per body: a 3 MB budget holds ~20k bodies, i.e. two 8192 systems
-    srv = khrr.Server(ctx, 3_000_000, 16)
-    reqs = [dict(arrays=_plummer_req(8192, 90 + k), steps=4) for k in range(4)]
-    ids = [srv.submit(*r["arrays"], 0.002, r["steps"], _grid()) for r in reqs]
-    done = srv.step()
-    st = srv.stats()
-    assert st["active"] == 2 and st["pending"] == 2 and not done
-    assert st["bytes_in_use"] <= st["bytes_budget"]
-    finished = []
-    for _ in range(100):
-        finished += srv.step()
-        if len(finished) == 4:
-            break
-    assert sorted(finished) == sorted(ids)
-    for rid in ids:
-        d0, d1, cm = srv.result(rid, 16 ** 3)
-        assert d0["M"] == pytest.approx(1.0) and cm.sum() == pytest.approx(1.0)
+    # an 8192-body system holds 152 B per body plus 4 j-pieces of partial
+    # sums (64 B per body): 1769472 B, so a 4 MB budget holds two of them
+    srv = khrr.Server(ctx, 4_000_000, 16)
+    try:
+        reqs = [dict(arrays=_plummer_req(8192, 90 + k), steps=4) for k in range(4)]
+        ids = [srv.submit(*r["arrays"], 0.002, r["steps"], _grid()) for r in reqs]
+        done = srv.step()
+        st = srv.stats()
+        assert st["active"] == 2 and st["pending"] == 2 and not done
+        assert st["bytes_in_use"] == 2 * 1769472 <= st["bytes_budget"]
+        finished = []
+        for _ in range(100):
+            finished += srv.step()
+            if len(finished) == 4:
+                break
+        assert sorted(finished) == sorted(ids)
+        for rid in ids:
+            d0, d1, cm = srv.result(rid, 16 ** 3)
+            assert d0["M"] == pytest.approx(1.0) and cm.sum() == pytest.approx(1.0)
+    finally:
+        srv.close()
+
+
+def test_server_outlived_by_nothing(ctx):
+    # closing a context closes its servers first; a server closed that way
+    # can be closed again harmlessly
+    import khrr
+    c = khrr.Context(0)
+    srv = khrr.Server(c, 1 << 24, 2)
+    srv.submit(*_plummer_req(1024, 1), 0.002, 2, _grid())
+    c.close()
     srv.close()
+    assert srv._h is None
 
 
 def test_errors(ctx):
