@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--batched-steps", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-lambda", type=float, default=3.0, help="arrivals per second")
+    ap.add_argument("--legs", default="binning,stencil,batched,serving,precision",
+                    help="extras to run after the headline (comma list)")
     ap.add_argument("--precision-steps", type=int, default=3,
                     help="steps of the 3.1M-body encounter per sum mode (0: skip)")
     ap.add_argument("--shape", default=None,
@@ -157,8 +159,11 @@ def bench_precision(ctx, steps, samples=8192):
         acc0 = sim.acc.cpu().numpy()[:, rows].astype(np.float64)
         e = np.linalg.norm(acc0 - ref, axis=0) / np.linalg.norm(ref, axis=0)
         d0 = sim.diagnostics()
-        for _ in range(steps):
+        for k in range(steps):
             sim.step()
+            if steps >= 50 and (k + 1) % 25 == 0:
+                torch.cuda.synchronize()
+                log(f"precision {mode}: step {k + 1}/{steps}")
         d1 = sim.diagnostics()
         ctx.set_timing(False)
         torch.cuda.synchronize()
@@ -260,7 +265,9 @@ def bench_batched(ctx, steps):
     tune, _ = ctx.tune_batched(off_list)
 
     def grav_batched(out):
-        ctx.accel_batched(off_dev, off_list, sim.pos32, sim.h32, sim.m32, acc=out, tune=tune)
+        # same composition and buffers every step: captured once, then replayed
+        ctx.accel_batched(off_dev, off_list, sim.pos32, sim.h32, sim.m32, acc=out, tune=tune,
+                          graph=True)
 
     # each system in its own buffers, as a one-system-per-process run holds it
     per_sys = []
@@ -302,7 +309,7 @@ def bench_batched(ctx, steps):
     return {"systems": 512, "bodies": N, "pairs_per_step": pairs,
             "batched_ms": t_b * 1e3, "plain_ms": t_p * 1e3,
             "batched_Gpairs_per_s": pairs / t_b / 1e9, "plain_Gpairs_per_s": pairs / t_p / 1e9,
-            "speedup_vs_plain": t_p / t_b, "tune": tune.as_dict(),
+            "speedup_vs_plain": t_p / t_b, "tune": tune.as_dict(), "graph": True,
             "plain": "khrr_accel per system, default launch shape, one stream",
             "plain_reference_shape_ms": t_r * 1e3,
             "speedup_vs_plain_reference_shape": t_r / t_b,
@@ -592,25 +599,29 @@ def main():
             cpu, acc = cpu_baseline(sim, chk.cpu().numpy(), args.cpu_rows)
             result["cpu_baseline"] = cpu
             result["accuracy"] = acc
-    if not args.quick:
+    legs = set() if args.quick else set(args.legs.split(","))
+    if legs:
         ctx.set_timing(True)
+    if legs & {"binning", "stencil"}:
         log("binning")
         cell_mass, binning = bench_binning(ctx, sim)
         result["binning"] = binning
-        log("stencil")
-        result["stencil"] = bench_stencil(ctx, cell_mass)
+        if "stencil" in legs:
+            log("stencil")
+            result["stencil"] = bench_stencil(ctx, cell_mass)
         del cell_mass
         torch.cuda.empty_cache()
+    if "batched" in legs:
         log("batched")
         result["batched"] = bench_batched(ctx, args.batched_steps)
         torch.cuda.empty_cache()
-        if args.serve_requests > 0:
-            log("serving")
-            result["serving"] = bench_serving(ctx, args.serve_requests, args.serve_lambda)
-            torch.cuda.empty_cache()
-        if args.precision_steps > 0:
-            log("precision form")
-            result["precision"] = bench_precision(ctx, args.precision_steps)
+    if "serving" in legs and args.serve_requests > 0:
+        log("serving")
+        result["serving"] = bench_serving(ctx, args.serve_requests, args.serve_lambda)
+        torch.cuda.empty_cache()
+    if "precision" in legs and args.precision_steps > 0:
+        log("precision form")
+        result["precision"] = bench_precision(ctx, args.precision_steps)
     print(json.dumps(result), flush=True)
 
 
