@@ -230,7 +230,9 @@ void orc_diag(int n, const double* pos, const double* vel, const double* mass, c
 /* nbody.cu:1468-1594: initial PHI+ACCEL, then per step predictor ->
  * ACCEL(x*) -> corrector. */
 void orc_run(int n, double* pos, double* vel, const double* mass, const double* h, double dt,
-             int steps, const orc_extpot* ext, int compat_accumulate, double* diag_out) {
+             int steps, const orc_extpot* ext, int compat_accumulate, double* diag_out,
+             int diag_every) {
+  if (diag_every < 1) diag_every = 1;
   const size_t N = (size_t)n;
   double* acc = (double*)calloc(3 * N, sizeof(double));
   double* acct = (double*)calloc(3 * N, sizeof(double));
@@ -245,9 +247,9 @@ void orc_run(int n, double* pos, double* vel, const double* mass, const double* 
     orc_accel(n, post, h, n, post, mass, h, acct, NULL, compat_accumulate);
     orc_integrate(n, 1, dt, tgrav, ext, acct, pos, vel, post, velt, acc);
     tgrav = (s + 2) * dt;
-    if (diag_out) {
+    if (diag_out && (s + 1) % diag_every == 0) {
       orc_accel(n, pos, h, n, pos, mass, h, NULL, phi, 0);
-      orc_diag(n, pos, vel, mass, phi, diag_out + (size_t)(s + 1) * 17);
+      orc_diag(n, pos, vel, mass, phi, diag_out + (size_t)((s + 1) / diag_every) * 17);
     }
   }
   free(acc);

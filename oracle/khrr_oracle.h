@@ -76,9 +76,11 @@ void orc_diag(int n, const double* pos, const double* vel, const double* mass, c
  * state (pos, vel: vec3 planes, updated in place).  compat_accumulate != 0
  * reproduces the never-reset force accumulator of nbody.cu:1560.
  * diag_out (may be NULL) receives 17 diagnostics at t=0 and after every
- * step: (steps+1)*17 doubles; phi for EP is recomputed at those points. */
+ * diag_every-th step: (steps/diag_every+1)*17 doubles; phi for EP is
+ * recomputed at those points. */
 void orc_run(int n, double* pos, double* vel, const double* mass, const double* h, double dt,
-             int steps, const orc_extpot* ext, int compat_accumulate, double* diag_out);
+             int steps, const orc_extpot* ext, int compat_accumulate, double* diag_out,
+             int diag_every);
 
 /* p2mesh binning p2mesh-kernel.cuh:11-137 with the deterministic choices
  * of include/khrr.h (index-order id_in_cell, per-cell and block-granular

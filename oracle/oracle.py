@@ -41,7 +41,7 @@ def lib():
         L.orc_integrate.argtypes = [_I, _I, _D, _D, ctypes.POINTER(OrcExtPot), _P, _P, _P, _P,
                                     _P, _P]
         L.orc_diag.argtypes = [_I, _P, _P, _P, _P, _P]
-        L.orc_run.argtypes = [_I, _P, _P, _P, _P, _D, _I, ctypes.POINTER(OrcExtPot), _I, _P]
+        L.orc_run.argtypes = [_I, _P, _P, _P, _P, _D, _I, ctypes.POINTER(OrcExtPot), _I, _P, _I]
         L.orc_p2mesh_bin.argtypes = [_P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]
         L.orc_p2mesh_bin.restype = _I
         L.orc_p2mesh_cell_phi.argtypes = [_I, ctypes.c_longlong, _P, _P, _P, _P, _I]
@@ -126,14 +126,16 @@ def diag(pos, vel, mass, phi=None):
     return out
 
 
-def run(pos, vel, mass, h, dt, steps, ext=None, compat_accumulate=False, with_diag=True):
-    """Reference step loop; returns (pos, vel, diag[(steps+1), 17])."""
+def run(pos, vel, mass, h, dt, steps, ext=None, compat_accumulate=False, with_diag=True,
+        diag_every=1):
+    """Reference step loop; returns (pos, vel, diag[(steps // diag_every + 1), 17]),
+    diagnostics at t = 0 and after every diag_every-th step."""
     pos, vel = _f64(pos).copy(), _f64(vel).copy()
     mass, h = _f64(mass), _f64(h)
-    d = np.zeros((steps + 1, 17)) if with_diag else None
+    d = np.zeros((steps // diag_every + 1, 17)) if with_diag else None
     lib().orc_run(pos.shape[1], _p(pos), _p(vel), _p(mass), _p(h), dt, steps,
                   ctypes.byref(ext) if ext is not None else None, 1 if compat_accumulate else 0,
-                  _p(d))
+                  _p(d), diag_every)
     return pos, vel, d
 
 

@@ -57,6 +57,29 @@ def test_kepler_orbit_against_mpmath_golden():
     assert np.max(np.abs(d[:, 2:5])) < 1e-15
 
 
+def test_c0_drift_fixture_is_the_oracle_run():
+    # the committed 500-step drift fixture (make_drift_golden.py) holds the
+    # oracle's own run of these bodies: same initial conditions, and its
+    # first two rows (t = 0 and after 10 steps) equal a fresh run bit for bit
+    g = np.load(os.path.join(GOLDEN, "c0_drift_golden.npz"))
+    from helpers import galaxy_system
+    pos, vel, mass, h = galaxy_system(int(g["n_s"]), int(g["n_dm"]), seed=int(g["seed"]))
+    m32 = mass.astype(np.float32).astype(np.float64)
+    h32 = h.astype(np.float32).astype(np.float64)
+    chk = np.array([pos.sum(), vel.sum(), (pos * pos).sum(), (vel * vel).sum(), m32.sum(),
+                    h32.sum()])
+    np.testing.assert_array_equal(chk, g["ic_checksum"])
+    every = int(g["every"])
+    _, _, d = orc.run(pos, vel, m32, h32, float(g["dt"]), every, diag_every=every)
+    np.testing.assert_array_equal(d, g["diag"][:2])
+    # the reference method conserves what it should over the 500 steps:
+    # energy to the integrator's error, momentum to FP64 rounding
+    E = 0.5 * (g["diag"][:, 0] + g["diag"][:, 1])
+    assert np.max(np.abs(E / E[0] - 1)) < 1e-3
+    p_scale = float(np.sum(m32 * np.linalg.norm(vel, axis=0)))
+    assert np.max(np.abs(g["diag"][:, 2:5] - g["diag"][0, 2:5])) < 1e-12 * p_scale
+
+
 def test_accumulator_reset_vs_compat_mode():
     # the never-reset force accumulator of nbody.cu:1560 blows up energy
     rng = np.random.default_rng(1)
