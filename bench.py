@@ -341,27 +341,38 @@ def serving_requests(n_req, seed=7, tmax_lo=0.05, tmax_hi=0.5, dt=0.002):
     return reqs
 
 
-def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4e10):
+def poisson_arrivals(n_req, lam, seed=8):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req))
+    return arrivals - arrivals[0]
+
+
+def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4e10,
+                  lam_saturated=50.0):
     """Continuous batching on a Poisson arrival stream vs the same requests
-    run one at a time (FIFO) with plain per-system launches."""
+    run one at a time (FIFO) with plain per-system launches; then the same
+    requests arriving faster than either can serve them (lam_saturated)."""
     import numpy as np
     import torch
     import khrr
     reqs = serving_requests(n_req)
-    rng = np.random.default_rng(8)
-    arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req))
-    arrivals -= arrivals[0]
+    arrivals = poisson_arrivals(n_req, lam)
     grid = khrr.cube_grid(128, -2.0, 2.0)
     cells = 128 ** 3
     pairs = sum(r["n"] ** 2 * (r["steps"] + 3) for r in reqs)  # init acc+phi, steps, final phi
 
-    def served(round_pairs):
-        """Real-time arrivals into one server; returns latencies and figures."""
+    def served(round_pairs, arrivals=arrivals):
+        """Real-time arrivals into one server; returns latencies and figures.
+        busy: wall time while the server holds work (from the step that finds
+        it non-empty to the step that drains it; a draining step finishes a
+        request and so waits for the device)."""
         srv = khrr.Server(ctx, int(budget_gb * 1e9), max_active, round_pairs=round_pairs)
         ctx.reset_timing()
         ctx.set_timing(True)
         ids, lat = {}, {}
         busy = 0.0
+        busy_since = None
         t0 = time.perf_counter()
         nxt = 0
         while len(lat) < n_req:
@@ -375,13 +386,17 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4
             if st["active"] == 0 and st["pending"] == 0:
                 time.sleep(max(0.0, min(0.01, arrivals[nxt] - now)) if nxt < n_req else 0.0)
                 continue
-            ts = time.perf_counter()
+            if busy_since is None:
+                busy_since = time.perf_counter()
             finished = srv.step()
-            busy += time.perf_counter() - ts
             for fid in finished:
                 k = ids[fid]
                 lat[k] = time.perf_counter() - t0 - arrivals[k]
                 srv.result(fid, cells)
+            st = srv.stats()
+            if st["active"] == 0 and st["pending"] == 0:
+                busy += time.perf_counter() - busy_since
+                busy_since = None
         makespan = time.perf_counter() - t0
         st = srv.stats()
         srv.close()
@@ -425,12 +440,30 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4
         service.append(time.perf_counter() - ts)
         del sim
     service = np.array(service)
-    done = np.zeros(n_req)
-    for k in range(n_req):  # FIFO queue with the same arrivals
-        done[k] = max(arrivals[k], done[k - 1] if k else 0.0) + service[k]
-    plain_lat = done - arrivals
+
+    def fifo_latency(arr):
+        done = np.zeros(n_req)
+        for k in range(n_req):  # FIFO queue with the same arrivals
+            done[k] = max(arr[k], done[k - 1] if k else 0.0) + service[k]
+        return done - arr
+
+    plain_lat = fifo_latency(arrivals)
     log(f"plain FIFO: busy {service.sum():.2f} s, p99 {np.percentile(plain_lat, 99):.3f} s")
     head = policies["least_work_first"]
+    # saturated: every request queued within a fraction of a second
+    arr_sat = poisson_arrivals(n_req, lam_saturated)
+    lat_sat, fig_sat = served(round_pairs, arr_sat)
+    plain_sat = fifo_latency(arr_sat)
+    saturated = {"lambda_per_s": lam_saturated, "policy": "least_work_first",
+                 "served_busy_s": fig_sat["busy_s"], "served_Gpairs_per_s": fig_sat["Gpairs_per_s"],
+                 "p50_latency_s": fig_sat["p50_latency_s"],
+                 "p99_latency_s": fig_sat["p99_latency_s"], "rounds": fig_sat["rounds"],
+                 "plain_p50_latency_s": float(np.percentile(plain_sat, 50)),
+                 "plain_p99_latency_s": float(np.percentile(plain_sat, 99)),
+                 "throughput_vs_plain": float(service.sum() / fig_sat["busy_s"])}
+    log(f"served (saturated, lambda {lam_saturated}/s): busy {fig_sat['busy_s']:.2f} s, "
+        f"p50 {fig_sat['p50_latency_s']:.3f} s, p99 {fig_sat['p99_latency_s']:.3f} s vs FIFO "
+        f"p50 {saturated['plain_p50_latency_s']:.3f} s, p99 {saturated['plain_p99_latency_s']:.3f} s")
     return {"requests": n_req, "lambda_per_s": lam, "tmax_range": [0.05, 0.5], "dt": 0.002,
             "grid": "128^3 over [-2,2]^3", "budget_GB": budget_gb, "max_active": max_active,
             "mix": "90% Plummer N in [1024,131072] log-uniform, 10% galaxies 65536+131072",
@@ -443,7 +476,8 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4
             "plain_Gpairs_per_s": pairs / service.sum() / 1e9,
             "plain_p50_latency_s": float(np.percentile(plain_lat, 50)),
             "plain_p99_latency_s": float(np.percentile(plain_lat, 99)),
-            "throughput_vs_plain": float(service.sum() / head["busy_s"])}
+            "throughput_vs_plain": float(service.sum() / head["busy_s"]),
+            "saturated": saturated}
 
 
 def cpu_baseline(sim, acc_gpu, rows_n):

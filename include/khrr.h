@@ -110,7 +110,7 @@ typedef struct {
                        evaluate each unordered pair once and apply it to
                        both bodies (needs i_per_lane 4 or 8 and a partial
                        buffer of 16 B per body per superblock -- 4096
-                       bodies from n = 131072 on, 2048 below -- at most
+                       bodies from n = 524288 on, 2048 below -- at most
                        40 GiB; otherwise the one-sided kernel runs)      */
 } khrr_tune;
 
@@ -137,11 +137,17 @@ int khrr_accel(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int
  *  - n >= 32768 (when its partial slots fit, as for khrr_accel): the
  *    pair-symmetric path, one workgroup per (A, B) superblock item; the
  *    result is bit-identical to khrr_accel with i_per_lane 8, symmetric 1;
- *  - otherwise the j-range is cut into k = min(16, ceil(n / 2048)) pieces
+ *  - otherwise the j-range is cut into k = min(32, ceil(n / 1024)) pieces
  *    and (i-group, j-piece) items run longest system first in one
  *    persistent launch, pieces summed in piece order; the result is
  *    bit-identical to khrr_accel with j_split k, j_tile 256, symmetric 0.
- * So a system's result never depends on what else is in the batch. */
+ * So a system's result never depends on what else is in the batch.  When
+ * both kinds are present the one-sided launch runs on a second stream of
+ * the context, concurrently with the pair-symmetric items, and the call's
+ * stream waits for it before returning control of its outputs.
+ * tune == NULL: the shape khrr_tune_batched stored in the context, else
+ * i_per_lane the widest of 4, 2, 1 that gives at least two one-sided
+ * items per SIMD (results do not depend on i_per_lane). */
 int khrr_accel_batched(khrr_ctx* ctx, int n_sys, const int* sys_off, const int* host_sys_off,
                        const float* pos, const float* h, const float* m, float* acc, float* phi,
                        unsigned flags, const khrr_tune* tune);
@@ -295,7 +301,9 @@ int khrr_mesh_to_particles(khrr_ctx* ctx, int n, long long n_cells, const int* c
  * keeps every admitted simulation's state resident in device memory and
  * advances all of them together, one predictor -> gravity -> corrector
  * step per khrr_server_step, with a single batched gravity launch over
- * every active system (khrr_accel_batched's per-system rules).  Requests
+ * every active system (khrr_accel_batched's per-system rules; each round
+ * picks i_per_lane for its own composition as khrr_accel_batched does
+ * with a NULL tune, starting from the tuned one).  Requests
  * that arrive while others run are admitted
  * (first come, first served) as soon as device memory within the budget
  * and an active slot are free.  A finished simulation returns its
@@ -326,7 +334,7 @@ typedef struct {
 /* hbm_budget: device bytes the server may hold for simulation state: a
  * simulation of n bodies holds 152 B per body (FP32 gravity arrays and FP64
  * state) plus the batched gravity's partial sums for it: 16 B per body per
- * superblock when n >= 32768 (2048 bodies below 131072, 4096 from there
+ * superblock when n >= 32768 (2048 bodies below 524288, 4096 from there
  * on), else 16 B per body per j-piece when there are two or more.  The
  * FP32 arena for budget / 152 bodies is reserved at creation.
  * max_active: simultaneous simulations (>= 1). */
@@ -347,7 +355,10 @@ int khrr_server_submit(khrr_server* srv, const khrr_request* req, long long* id)
 /* One scheduling round: admit what fits, advance every active simulation
  * by one step, finish those that reached their step count.  Writes up to
  * max_done finished ids to done_ids (host) and their number to *n_done.
- * Synchronous. */
+ * A round that admits or finishes a simulation returns when the device has
+ * completed it (its results are then ready); any other round returns once
+ * its launches are queued, with at most two rounds in flight, so the host
+ * prepares the next round while the device runs this one. */
 int khrr_server_step(khrr_server* srv, long long* done_ids, int max_done, int* n_done);
 /* Copies a finished simulation's results and releases it.  diag0/diag1:
  * KHRR_DIAG_COUNT doubles each (t = 0 and final; EP uses the potential);

@@ -37,10 +37,10 @@ struct RedItem {
 
 // Number of j-pieces of a system of n bodies: a rule of the system's own
 // size, so the summation order never depends on the rest of the batch.
-// Pieces of about 2048 j-bodies keep a wave's item short (256 i x 2048 j at
-// i_per_lane 4), so a round of a few small systems is not held up by a few
-// long waves.
-inline int pieces_of(int n) { return std::max(1, std::min(16, (n + 2047) / 2048)); }
+// Pieces of about 1024 j-bodies keep a wave's item short (2048 serial pair
+// steps per lane at i_per_lane 2), so a round of a few small systems is not
+// held up by a few long waves: such a round lasts about one item.
+inline int pieces_of(int n) { return std::max(1, std::min(32, (n + 1023) / 1024)); }
 
 template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
 __global__ __launch_bounds__(256) void batched_kernel(
@@ -148,10 +148,14 @@ __global__ __launch_bounds__(256) void batched_reduce(const RedItem* __restrict_
 // range [tiles[t].x, tiles[t].y) inside one system) are bitwise equal.  The
 // tiles of a system start at its first body, exactly as the plain kernel's
 // tiles start at j = 0, so both forms take the same arithmetic path.
+// It also clears the persistent grid's item counter, which every launch
+// sequence runs after it.
 __global__ __launch_bounds__(256) void tile_flags_kernel(int n_tiles, const int2* __restrict__ tiles,
                                                          const float* __restrict__ h,
                                                          const float* __restrict__ m,
-                                                         int* __restrict__ flags) {
+                                                         int* __restrict__ flags,
+                                                         int* __restrict__ counter) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
   const int lane = threadIdx.x & (khrr::kWave - 1);
   const int t = blockIdx.x * (blockDim.x / khrr::kWave) + threadIdx.x / khrr::kWave;
   if (t >= n_tiles) return;
@@ -168,8 +172,6 @@ __global__ __launch_bounds__(256) void tile_flags_kernel(int n_tiles, const int2
   if (lane == 0)
     flags[t] = (bad_h == 0ull ? khrr::kUniformH : 0) | (bad_m == 0ull ? khrr::kUniformM : 0);
 }
-
-__global__ void set_word(int* __restrict__ p, int v) { *p = v; }
 
 using BatchLauncher = void (*)(dim3, dim3, hipStream_t, int, const Item*, const int*, int,
                                const float*, const float*, const float*, int, int*, float*, float*,
@@ -435,52 +437,92 @@ int blocks_per_cu(int ipl, bool a, bool p, bool c, int wpb) {
   return v;
 }
 
-// The launches of one batched evaluation over a prepared work list: tile
-// flags, the pair-symmetric items and their slot reduction, the persistent
-// one-sided grid and the piece reduction.  No host synchronisation and no
-// allocation, so the sequence can be captured into a graph.
-int launch_work(khrr_ctx* ctx, const WorkList& wl, int ld, const float* pos, const float* h,
-                const float* m, float* acc, float* phi, unsigned flags, const khrr_tune& t,
-                float* part, float* sym_part) {
+// The persistent one-sided grid over the work list's (i-group, j-piece)
+// items on stream `st`: as many blocks as stay resident, never more waves
+// than items; then the piece reduction of the cut systems.
+int launch_one_sided(khrr_ctx* ctx, hipStream_t st, const WorkList& wl, int ld, const float* pos,
+                     const float* h, const float* m, float* acc, float* phi, bool comp,
+                     int accumulate, const khrr_tune& t, float* part) {
   using namespace khrr;
-  const bool comp = (flags & KHRR_COMPENSATED) != 0;
-  const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
-  if (wl.n_tiles > 0)
-    hipLaunchKernelGGL(tile_flags_kernel, dim3(ceil_div(wl.n_tiles, 4)), dim3(256), 0, ctx->stream,
-                       wl.n_tiles, wl.tiles, h, m, wl.tile_flags);
-  KHRR_TRY(check_launch(ctx, "tile_flags_kernel"));
-  // pair-symmetric systems first (one workgroup per (A, B) item) ...
-  KHRR_TRY(sym_items_launch(ctx, wl.n_sym_big, wl.n_sym_items - wl.n_sym_big, wl.sym_items,
-                            wl.n_sym_sys, wl.sym_sys, wl.max_sym_n, ld, pos, h, m, wl.tile_flags,
-                            sym_part, acc, phi, flags));
-  if (wl.n_items == 0) return 0;
-  // ... then the persistent one-sided grid: as many blocks as stay resident,
-  // never more waves than items
   BatchLauncher fn = pick_batched(t.i_per_lane, acc != nullptr, phi != nullptr, comp);
   const int wpb = t.waves_per_block;
   const long long resident =
       (long long)blocks_per_cu(t.i_per_lane, acc != nullptr, phi != nullptr, comp, wpb) *
       ctx->cu_count;
   const long long blocks = std::min<long long>(resident, ceil_div(wl.n_items, wpb));
-  int* counter = ctx->dev_words;
-  hipLaunchKernelGGL(set_word, dim3(1), dim3(1), 0, ctx->stream, counter, 0);
-  fn(dim3((unsigned)blocks), dim3(64 * wpb), ctx->stream, wl.n_items, wl.items, wl.tile_flags, ld,
-     pos, h, m, t.j_tile, counter, acc, phi, part, accumulate);
+  int* counter = ctx->dev_words;  // cleared by tile_flags_kernel
+  fn(dim3((unsigned)blocks), dim3(64 * wpb), st, wl.n_items, wl.items, wl.tile_flags, ld, pos, h,
+     m, t.j_tile, counter, acc, phi, part, accumulate);
   if (wl.n_red > 0) {
     const dim3 grid(ceil_div(wl.max_red_count, 256), wl.n_red);
     if (comp)
-      hipLaunchKernelGGL(batched_reduce<true>, grid, dim3(256), 0, ctx->stream, wl.red, ld,
+      hipLaunchKernelGGL(batched_reduce<true>, grid, dim3(256), 0, st, wl.red, ld,
                          (int)(acc != nullptr), (int)(phi != nullptr), part, acc, phi, accumulate);
     else
-      hipLaunchKernelGGL(batched_reduce<false>, grid, dim3(256), 0, ctx->stream, wl.red, ld,
+      hipLaunchKernelGGL(batched_reduce<false>, grid, dim3(256), 0, st, wl.red, ld,
                          (int)(acc != nullptr), (int)(phi != nullptr), part, acc, phi, accumulate);
   }
   return check_launch(ctx, "batched_kernel");
 }
 
+// The launches of one batched evaluation over a prepared work list: tile
+// flags, then the pair-symmetric items with their slot reduction and the
+// one-sided grid with its piece reduction.  The two parts touch disjoint
+// systems, so when both are present the one-sided part runs on the
+// context's second stream (forked after the flags, joined at the end) and
+// fills the CUs the symmetric launch leaves idle near its end.  No host
+// synchronisation and no allocation, so the sequence can be captured into
+// a graph (the fork and join become graph edges).
+int launch_work(khrr_ctx* ctx, const WorkList& wl, int ld, const float* pos, const float* h,
+                const float* m, float* acc, float* phi, unsigned flags, const khrr_tune& t,
+                float* part, float* sym_part) {
+  using namespace khrr;
+  const bool comp = (flags & KHRR_COMPENSATED) != 0;
+  const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
+  if (wl.n_tiles > 0)  // every system with items has tiles
+    hipLaunchKernelGGL(tile_flags_kernel, dim3(ceil_div(wl.n_tiles, 4)), dim3(256), 0, ctx->stream,
+                       wl.n_tiles, wl.tiles, h, m, wl.tile_flags, ctx->dev_words);
+  KHRR_TRY(check_launch(ctx, "tile_flags_kernel"));
+  const bool fork = wl.n_items > 0 && wl.n_sym_items > 0;
+  if (fork) {
+    KHRR_TRY(aux_stream(ctx));
+    KHRR_HIP(ctx, hipEventRecord(ctx->fork_event, ctx->stream));
+    KHRR_HIP(ctx, hipStreamWaitEvent(ctx->aux_stream, ctx->fork_event, 0));
+    KHRR_TRY(launch_one_sided(ctx, ctx->aux_stream, wl, ld, pos, h, m, acc, phi, comp, accumulate,
+                              t, part));
+    KHRR_HIP(ctx, hipEventRecord(ctx->join_event, ctx->aux_stream));
+  }
+  KHRR_TRY(sym_items_launch(ctx, wl.n_sym_big, wl.n_sym_items - wl.n_sym_big, wl.sym_items,
+                            wl.n_sym_sys, wl.sym_sys, wl.max_sym_n, ld, pos, h, m, wl.tile_flags,
+                            sym_part, acc, phi, flags));
+  if (fork) {
+    KHRR_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->join_event, 0));
+    return 0;
+  }
+  if (wl.n_items == 0) return 0;
+  return launch_one_sided(ctx, ctx->stream, wl, ld, pos, h, m, acc, phi, comp, accumulate, t,
+                          part);
+}
+
 }  // namespace
 
 namespace khrr {
+
+int round_ipl(const khrr_ctx* ctx, int n_sys, const int* count, int pref) {
+  const long long want = 2ll * 4 * ctx->cu_count;  // two items per SIMD
+  bool any = false;
+  for (int ipl = pref; ipl >= 1; ipl /= 2) {
+    long long items = 0;
+    for (int s = 0; s < n_sys; ++s) {
+      const int n = count[s];
+      if (n <= 0 || on_sym_path(n)) continue;
+      any = true;
+      items += (long long)ceil_div(n, 64ll * ipl) * pieces_of(n);
+    }
+    if (!any || items >= want) return ipl;
+  }
+  return 1;
+}
 
 size_t batched_part_bytes(int n, bool compensated) {
   if (n <= 0) return 0;
@@ -671,12 +713,16 @@ int khrr_accel_batched(khrr_ctx* ctx, int n_sys, const int* sys_off, const int* 
   if (n_sys > 0 && host_sys_off[n_sys] > 0 && (!pos || !h || !m))
     return khrr::fail(ctx, KHRR_EINVAL, "null input pointer");
   khrr_tune t;
-  if (tune)
+  if (tune) {
     t = *tune;
-  else if (ctx->has_batch_tuned)
+  } else if (ctx->has_batch_tuned) {
     t = ctx->batch_tuned;
-  else
-    t = khrr_tune{4, 4, 1, 256};
+  } else {
+    t = khrr_tune{4, 4, 1, 256, 0};
+    std::vector<int> count(n_sys);
+    for (int s = 0; s < n_sys; ++s) count[s] = host_sys_off[s + 1] - host_sys_off[s];
+    t.i_per_lane = khrr::round_ipl(ctx, n_sys, count.data(), 4);
+  }
   t.j_split = 1;
   KHRR_TRY(khrr::validate_tune(ctx, t));
   (void)sys_off;  // the host offsets define the segments

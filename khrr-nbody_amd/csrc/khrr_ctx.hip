@@ -53,6 +53,14 @@ int scratch(khrr_ctx* ctx, size_t bytes, void** base) {
   return 0;
 }
 
+int aux_stream(khrr_ctx* ctx) {
+  if (ctx->aux_stream) return 0;
+  KHRR_HIP(ctx, hipEventCreateWithFlags(&ctx->fork_event, hipEventDisableTiming));
+  KHRR_HIP(ctx, hipEventCreateWithFlags(&ctx->join_event, hipEventDisableTiming));
+  KHRR_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+  return 0;
+}
+
 int check_launch(khrr_ctx* ctx, const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
@@ -159,6 +167,12 @@ void khrr_ctx_destroy(khrr_ctx* ctx) {
     if (g.dev) (void)hipFree(g.dev);
   }
   if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
+  if (ctx->aux_stream) {
+    (void)hipStreamSynchronize(ctx->aux_stream);
+    (void)hipStreamDestroy(ctx->aux_stream);
+  }
+  if (ctx->fork_event) (void)hipEventDestroy(ctx->fork_event);
+  if (ctx->join_event) (void)hipEventDestroy(ctx->join_event);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->batch_dev) (void)hipFree(ctx->batch_dev);
   if (ctx->batch_pinned) (void)hipHostFree(ctx->batch_pinned);

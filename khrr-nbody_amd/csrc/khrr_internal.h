@@ -104,6 +104,11 @@ struct khrr_ctx {
   std::vector<khrr::GraphEntry> graphs;
   unsigned long long graph_clock = 0;
   hipStream_t capture_stream = nullptr;  // private, non-blocking, default priority
+  // Second stream for concurrent parts of one call (forked from and joined
+  // back to `stream` with the two events); non-blocking, default priority.
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t fork_event = nullptr;
+  hipEvent_t join_event = nullptr;
 };
 
 namespace khrr {
@@ -119,6 +124,9 @@ int enter(khrr_ctx* ctx);
 
 // Ensures the scratch arena holds at least `bytes`; returns its base.
 int scratch(khrr_ctx* ctx, size_t bytes, void** base);
+
+// Creates the context's second stream and its fork/join events on first use.
+int aux_stream(khrr_ctx* ctx);
 
 // Bump allocator over a scratch region (256-byte aligned pieces).
 struct Carver {
@@ -175,11 +183,17 @@ int integrate_segments(khrr_ctx* ctx, int n_seg, const IntSeg* segs, int total_b
 int batched_segments(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* host_count,
                      int ld, const float* pos, const float* h, const float* m, float* acc,
                      float* phi, unsigned flags, const khrr_tune& t);
+// i-bodies per lane for one batched evaluation of these systems: the
+// widest of pref, pref/2, ..., 1 whose one-sided items number at least two
+// per SIMD, else 1, so a round of a few small systems lasts about one
+// short item.  Results do not depend on it (each body's j order is fixed
+// by its system's j-pieces).
+int round_ipl(const khrr_ctx* ctx, int n_sys, const int* count, int pref);
 // Scratch bytes the batched form needs for the partial sums of one system
 // of n bodies (0 when it writes its outputs directly).
 size_t batched_part_bytes(int n, bool compensated);
 // Pair-symmetric form (khrr_sym.hip) for one system (i-set == j-set).
-// Bodies per superblock for a system of n bodies: 4096 from 131072 bodies
+// Bodies per superblock for a system of n bodies: 4096 from 524288 bodies
 // on, else 2048 (a rule of n alone, shared by the plain and batched forms).
 int sym_superblock(int n);
 bool sym_eligible(int n, int ipl);

@@ -17,6 +17,9 @@
 //      the round's composition repeats) and ONE corrector launch;
 //   3. systems that reached their step count get their potential, final
 //      diagnostics and the p2mesh mass grid, and release their memory.
+// Only rounds that admit or finish wait for the device; the others return
+// once queued (two segment tables in a ring bound the run-ahead to two
+// rounds), so host scheduling overlaps the device's work.
 // A system's arithmetic never mixes with another's (segment-local j loops,
 // per-system flags, per-system reductions), so its results are
 // bit-identical to running it alone.
@@ -127,8 +130,15 @@ struct khrr_server {
   // Round scheduling: least remaining work first until the round holds
   // round_pairs pair-interactions per step (0: every active system).
   double round_pairs = 0.0;
-  khrr::IntSeg* seg_dev = nullptr;  // integrator segment table (max_active)
-  khrr::IntSeg* seg_pin = nullptr;
+  // Integrator segment tables (max_active entries each), a ring of two so
+  // the host can fill one while the device still reads the other; done[k]
+  // marks the end of the last round that used table k.
+  static constexpr int kRing = 2;
+  khrr::IntSeg* seg_dev[kRing] = {};
+  khrr::IntSeg* seg_pin[kRing] = {};
+  hipEvent_t done[kRing] = {};
+  bool recorded[kRing] = {};
+  long long ring_pos = 0;
 };
 
 namespace {
@@ -142,8 +152,10 @@ int gravity_over(khrr_server* s, const std::vector<Sim*>& set, float* acc, float
     begin.push_back(m->begin);
     count.push_back(m->n);
   }
+  khrr_tune t = s->tune;
+  t.i_per_lane = khrr::round_ipl(s->ctx, (int)set.size(), count.data(), s->tune.i_per_lane);
   return khrr::batched_segments(s->ctx, (int)set.size(), begin.data(), count.data(), s->cap,
-                                s->pos32, s->h32, s->m32, acc, phi, flags, s->tune);
+                                s->pos32, s->h32, s->m32, acc, phi, flags, t);
 }
 
 int upload(khrr_server* s, Sim* m) {
@@ -278,10 +290,18 @@ int khrr_server_create(khrr_ctx* ctx, size_t hbm_budget, int max_active, khrr_se
   p += 3 * cap;
   s->phi = p;
   s->ranges.init((int)cap);
-  if (hipMalloc(&s->seg_dev, sizeof(khrr::IntSeg) * max_active) != hipSuccess ||
-      hipHostMalloc(&s->seg_pin, sizeof(khrr::IntSeg) * max_active) != hipSuccess) {
+  bool ok = true;
+  for (int k = 0; k < khrr_server::kRing && ok; ++k)
+    ok = hipMalloc(&s->seg_dev[k], sizeof(khrr::IntSeg) * max_active) == hipSuccess &&
+         hipHostMalloc(&s->seg_pin[k], sizeof(khrr::IntSeg) * max_active) == hipSuccess &&
+         hipEventCreateWithFlags(&s->done[k], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
     (void)hipGetLastError();
-    if (s->seg_dev) (void)hipFree(s->seg_dev);
+    for (int k = 0; k < khrr_server::kRing; ++k) {
+      if (s->seg_dev[k]) (void)hipFree(s->seg_dev[k]);
+      if (s->seg_pin[k]) (void)hipHostFree(s->seg_pin[k]);
+      if (s->done[k]) (void)hipEventDestroy(s->done[k]);
+    }
     (void)hipFree(s->arena);
     delete s;
     return khrr::fail(ctx, KHRR_ENOMEM, "server tables for %d simulations", max_active);
@@ -300,8 +320,11 @@ void khrr_server_destroy(khrr_server* s) {
   for (auto& kv : s->sims)
     if (kv.second->dev) (void)hipFree(kv.second->dev);
   if (s->arena) (void)hipFree(s->arena);
-  if (s->seg_dev) (void)hipFree(s->seg_dev);
-  if (s->seg_pin) (void)hipHostFree(s->seg_pin);
+  for (int k = 0; k < khrr_server::kRing; ++k) {
+    if (s->seg_dev[k]) (void)hipFree(s->seg_dev[k]);
+    if (s->seg_pin[k]) (void)hipHostFree(s->seg_pin[k]);
+    if (s->done[k]) (void)hipEventDestroy(s->done[k]);
+  }
   if (s->bin_count) (void)hipFree(s->bin_count);
   if (s->bin_mass) (void)hipFree(s->bin_mass);
   delete s;
@@ -377,20 +400,28 @@ int khrr_server_step(khrr_server* s, long long* done_ids, int max_done, int* n_d
   }
   if (!stepping.empty()) {
     // predictor -> one batched gravity launch -> corrector (nbody.cu:1525-1582)
+    const int slot = (int)(s->ring_pos++ % khrr_server::kRing);
+    // the round that last used this table (two rounds ago) must be done
+    // before its pinned copy is overwritten
+    if (s->recorded[slot]) KHRR_HIP(ctx, hipEventSynchronize(s->done[slot]));
+    khrr::IntSeg* pin = s->seg_pin[slot];
+    khrr::IntSeg* seg = s->seg_dev[slot];
     int blocks = 0;
     for (size_t k = 0; k < stepping.size(); ++k) {
       Sim* m = stepping[k];
-      s->seg_pin[k] = khrr::IntSeg{m->dev, m->dt, m->n, m->begin, blocks, 0};
+      pin[k] = khrr::IntSeg{m->dev, m->dt, m->n, m->begin, blocks, 0};
       blocks += (int)khrr::ceil_div(m->n, 256);
     }
     const int n_seg = (int)stepping.size();
-    KHRR_HIP(ctx, hipMemcpyAsync(s->seg_dev, s->seg_pin, n_seg * sizeof(khrr::IntSeg),
-                                 hipMemcpyHostToDevice, ctx->stream));
-    KHRR_TRY(khrr::integrate_segments(ctx, n_seg, s->seg_dev, blocks, 0, s->acc, nullptr,
-                                      s->pos32, s->cap));
-    KHRR_TRY(gravity_over(s, stepping, s->acct, nullptr, KHRR_GRAPH));
-    KHRR_TRY(khrr::integrate_segments(ctx, n_seg, s->seg_dev, blocks, 1, s->acct, s->acc, nullptr,
+    KHRR_HIP(ctx, hipMemcpyAsync(seg, pin, n_seg * sizeof(khrr::IntSeg), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    KHRR_TRY(khrr::integrate_segments(ctx, n_seg, seg, blocks, 0, s->acc, nullptr, s->pos32,
                                       s->cap));
+    KHRR_TRY(gravity_over(s, stepping, s->acct, nullptr, KHRR_GRAPH));
+    KHRR_TRY(khrr::integrate_segments(ctx, n_seg, seg, blocks, 1, s->acct, s->acc, nullptr,
+                                      s->cap));
+    KHRR_HIP(ctx, hipEventRecord(s->done[slot], ctx->stream));
+    s->recorded[slot] = true;
     for (Sim* m : stepping) {
       ++m->done_steps;
       s->pairs += m->n * (double)m->n;
@@ -399,7 +430,9 @@ int khrr_server_step(khrr_server* s, long long* done_ids, int max_done, int* n_d
     ++s->rounds;
   }
   KHRR_TRY(finish(s, done, done_ids, max_done, n_done));
-  KHRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // rounds that only step return with their launches queued; admission and
+  // finishing have already waited for the device
+  if (!admitted.empty() || !done.empty()) KHRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return 0;
 }
 

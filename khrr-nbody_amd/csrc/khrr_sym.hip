@@ -7,8 +7,8 @@
 // transcendental, subtraction and r^2 work per directed pair.
 //
 // Work decomposition (gfx950):
-//  * bodies are cut into superblocks of SB bodies (4096 for n >= 131072,
-//    else 2048, so mid-size systems still make enough items); a workgroup
+//  * bodies are cut into superblocks of SB bodies (4096 for n >= 524288,
+//    else 2048, so smaller systems still make enough items); a workgroup
 //    of SB / (64 IPL) waves takes one item (A, B) of the upper triangle,
 //    A <= B;
 //  * off-diagonal items: each wave owns 64*IPL i-bodies of A; the j-bodies of
@@ -362,7 +362,12 @@ namespace khrr {
 // 786432 bodies and 38.6 GB at 3145728 (of 288 GB HBM).
 constexpr size_t kSymPartBudget = size_t(40) << 30;
 
-int sym_superblock(int n) { return n >= 131072 ? 4096 : 2048; }
+// 2048-body superblocks give four times the items of 4096 (and four
+// 4-wave workgroups per CU instead of two 8-wave ones), so a lone system
+// below a few 10^5 bodies leaves a smaller share of the launch to its last,
+// partly filled wave of workgroups; from 524288 bodies on the 4096 form has
+// items enough and half the partial-slot memory.
+int sym_superblock(int n) { return n >= 524288 ? 4096 : 2048; }
 
 bool sym_eligible(int n, int ipl) {
   if (ipl != 4 && ipl != 8) return false;

@@ -2,7 +2,7 @@
 in one persistent launch.  Each system's result must equal the plain
 single-system launch bit for bit, whatever else is in the batch: for
 n >= 32768 the pair-symmetric launch (symmetric 1, i_per_lane 8), below it
-the one-sided launch with j_split min(16, ceil(n/2048)), j_tile 256.  Results
+the one-sided launch with j_split min(32, ceil(n/1024)), j_tile 256.  Results
 must also match the FP64 oracle within the gravity tolerance (median <= 1e-6,
 p99 <= 1e-5)."""
 import numpy as np
@@ -44,7 +44,7 @@ def _plain_tune(n):
     import khrr
     if n >= 32768:
         return khrr.Tune(8, 4, 1, 256, 1)
-    return khrr.Tune(4, 4, max(1, min(16, -(-n // 2048))), 256, 0)
+    return khrr.Tune(4, 4, max(1, min(32, -(-n // 1024))), 256, 0)
 
 
 def _run_plain(ctx, pos, m, h, phi=False, compensated=False):
@@ -122,7 +122,7 @@ def test_composition_change_and_empty_systems(ctx):
 
 @pytest.mark.parametrize("compensated", [False, True])
 def test_large_systems_equal_plain(ctx, compensated):
-    # n < 32768: j-range cut into ~2048-body pieces summed in piece order;
+    # n < 32768: j-range cut into ~1024-body pieces summed in piece order;
     # n >= 32768: pair-symmetric superblock items; both agree exactly with
     # the plain launch of the same shape
     sizes = [40000, 1000, 16385, 70000, 16384, 32768, 32767]

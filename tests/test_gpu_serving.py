@@ -88,16 +88,16 @@ def test_energy_follows_oracle(ctx):
 
 def test_admission_respects_budget(ctx):
     import khrr
-    # an 8192-body system holds 152 B per body plus 4 j-pieces of partial
-    # sums (64 B per body): 1769472 B, so a 4 MB budget holds two of them
-    srv = khrr.Server(ctx, 4_000_000, 16)
+    # an 8192-body system holds 152 B per body plus 8 j-pieces of partial
+    # sums (128 B per body): 2293760 B, so a 5 MB budget holds two of them
+    srv = khrr.Server(ctx, 5_000_000, 16)
     try:
         reqs = [dict(arrays=_plummer_req(8192, 90 + k), steps=4) for k in range(4)]
         ids = [srv.submit(*r["arrays"], 0.002, r["steps"], _grid()) for r in reqs]
         done = srv.step()
         st = srv.stats()
         assert st["active"] == 2 and st["pending"] == 2 and not done
-        assert st["bytes_in_use"] == 2 * 1769472 <= st["bytes_budget"]
+        assert st["bytes_in_use"] == 2 * 2293760 <= st["bytes_budget"]
         finished = []
         for _ in range(100):
             finished += srv.step()
