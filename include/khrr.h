@@ -110,7 +110,8 @@ typedef struct {
                        evaluate each unordered pair once and apply it to
                        both bodies (needs i_per_lane 4 or 8 and a partial
                        buffer of 16 B per body per superblock -- 4096
-                       bodies from n = 524288 on, 2048 below -- at most
+                       bodies from n = 524288 on, 2048 from 32768, 1024
+                       below, which always runs 4 i per lane -- at most
                        40 GiB; otherwise the one-sided kernel runs)      */
 } khrr_tune;
 
@@ -134,9 +135,10 @@ int khrr_accel(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int
  * symmetry and may be NULL (the work list is built on the host).  pos/acc
  * are vec3 planes of n_total entries.  Per-system rules, functions of the
  * system's own n only:
- *  - n >= 32768 (when its partial slots fit, as for khrr_accel): the
+ *  - n >= 16384 (when its partial slots fit, as for khrr_accel): the
  *    pair-symmetric path, one workgroup per (A, B) superblock item; the
- *    result is bit-identical to khrr_accel with i_per_lane 8, symmetric 1;
+ *    result is bit-identical to khrr_accel with i_per_lane 8, symmetric 1
+ *    (superblocks of 1024 bodies below 32768, which run 4 i per lane);
  *  - otherwise the j-range is cut into k = min(32, ceil(n / 1024)) pieces
  *    and (i-group, j-piece) items run longest system first in one
  *    persistent launch, pieces summed in piece order; the result is
@@ -334,8 +336,9 @@ typedef struct {
 /* hbm_budget: device bytes the server may hold for simulation state: a
  * simulation of n bodies holds 152 B per body (FP32 gravity arrays and FP64
  * state) plus the batched gravity's partial sums for it: 16 B per body per
- * superblock when n >= 32768 (2048 bodies below 524288, 4096 from there
- * on), else 16 B per body per j-piece when there are two or more.  The
+ * superblock when n >= 16384 (1024 bodies below 32768, 2048 below 524288,
+ * 4096 from there on), else 16 B per body per j-piece when there are two
+ * or more.  The
  * FP32 arena for budget / 152 bodies is reserved at creation.
  * max_active: simultaneous simulations (>= 1). */
 int khrr_server_create(khrr_ctx* ctx, size_t hbm_budget, int max_active, khrr_server** out);

@@ -224,10 +224,11 @@ BatchLauncher pick_batched(int ipl, bool a, bool p, bool c) {
   }
 }
 
-// Systems of at least 32768 bodies take the pair-symmetric path (16 or
-// more superblocks: the one-sided diagonal items are at most 1/16 of the
-// system's work).  Like the piece count, a rule of the system's size.
-inline bool on_sym_path(int n) { return n >= 32768 && khrr::sym_eligible(n, 8); }
+// Systems of at least 16384 bodies take the pair-symmetric path (16 or
+// more superblocks of 1024, 2048 or 4096 bodies: the one-sided diagonal
+// items are at most 1/16 of the system's work).  Like the piece count, a
+// rule of the system's size.
+inline bool on_sym_path(int n) { return n >= 16384 && khrr::sym_eligible(n, 8); }
 
 // Device work list of one composition.
 struct WorkList {
@@ -241,6 +242,7 @@ struct WorkList {
   khrr::SymItem* sym_items = nullptr;
   int n_sym_items = 0;
   int n_sym_big = 0;
+  int n_sym_mid = 0;
   khrr::SymSys* sym_sys = nullptr;
   int n_sym_sys = 0;
   int max_sym_n = 0;
@@ -283,7 +285,7 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
   if (!same) {
     std::vector<int> flag_base(n_sys);
     std::vector<int2> tiles;
-    std::vector<khrr::SymItem> sym_big, sym_small;
+    std::vector<khrr::SymItem> sym_big, sym_mid, sym_small;
     std::vector<khrr::SymSys> sym_sys;
     long long sym_floats = 0;
     int max_sym = 0;
@@ -295,7 +297,7 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
       for (int jt = b; jt < e; jt += step) tiles.push_back(make_int2(jt, std::min(e, jt + step)));
       if (!SB) continue;
       const int n_sb = (n + SB - 1) / SB;
-      auto& dst = SB == 4096 ? sym_big : sym_small;
+      auto& dst = SB == 4096 ? sym_big : SB == 2048 ? sym_mid : sym_small;
       for (int A = 0; A < n_sb; ++A)
         for (int B = A; B < n_sb; ++B)
           dst.push_back(khrr::SymItem{b, n, A, B, flag_base[s], 0, sym_floats});
@@ -304,6 +306,7 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
       max_sym = std::max(max_sym, n);
     }
     std::vector<khrr::SymItem> sym_items(sym_big);
+    sym_items.insert(sym_items.end(), sym_mid.begin(), sym_mid.end());
     sym_items.insert(sym_items.end(), sym_small.begin(), sym_small.end());
     std::vector<int> order(n_sys);
     std::iota(order.begin(), order.end(), 0);
@@ -373,6 +376,7 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
     ctx->batch_units = unit;
     ctx->batch_sym_items = (int)sym_items.size();
     ctx->batch_sym_big = (int)sym_big.size();
+    ctx->batch_sym_mid = (int)sym_mid.size();
     ctx->batch_sym_sys = (int)sym_sys.size();
     ctx->batch_sym_max = max_sym;
     ctx->batch_sym_floats = sym_floats;
@@ -392,6 +396,7 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
   wl->part_units = ctx->batch_units;
   wl->n_sym_items = ctx->batch_sym_items;
   wl->n_sym_big = ctx->batch_sym_big;
+  wl->n_sym_mid = ctx->batch_sym_mid;
   wl->n_sym_sys = ctx->batch_sym_sys;
   wl->max_sym_n = ctx->batch_sym_max;
   wl->sym_floats = ctx->batch_sym_floats;
@@ -492,7 +497,8 @@ int launch_work(khrr_ctx* ctx, const WorkList& wl, int ld, const float* pos, con
                               t, part));
     KHRR_HIP(ctx, hipEventRecord(ctx->join_event, ctx->aux_stream));
   }
-  KHRR_TRY(sym_items_launch(ctx, wl.n_sym_big, wl.n_sym_items - wl.n_sym_big, wl.sym_items,
+  KHRR_TRY(sym_items_launch(ctx, wl.n_sym_big, wl.n_sym_mid,
+                            wl.n_sym_items - wl.n_sym_big - wl.n_sym_mid, wl.sym_items,
                             wl.n_sym_sys, wl.sym_sys, wl.max_sym_n, ld, pos, h, m, wl.tile_flags,
                             sym_part, acc, phi, flags));
   if (fork) {

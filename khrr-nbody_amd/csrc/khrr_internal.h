@@ -92,7 +92,8 @@ struct khrr_ctx {
   int batch_red_max = 0;      // largest such system
   long long batch_units = 0;  // partial-sum rows of all cut systems
   int batch_sym_items = 0;    // pair-symmetric (A, B) items
-  int batch_sym_big = 0;      // of which the first ones have 4096-body superblocks
+  int batch_sym_big = 0;      // of which the first ones have 4096-body superblocks,
+  int batch_sym_mid = 0;      // the next ones 2048 (the rest 1024)
   int batch_sym_sys = 0;      // systems on the pair-symmetric path
   int batch_sym_max = 0;      // largest such system
   long long batch_sym_floats = 0;  // their partial slots
@@ -196,6 +197,9 @@ size_t batched_part_bytes(int n, bool compensated);
 // Bodies per superblock for a system of n bodies: 4096 from 524288 bodies
 // on, else 2048 (a rule of n alone, shared by the plain and batched forms).
 int sym_superblock(int n);
+// i-bodies per lane the pair-symmetric kernel runs for n bodies when asked
+// for ipl (1024-body superblocks: always 4).
+int sym_ipl(int n, int ipl);
 bool sym_eligible(int n, int ipl);
 bool sym_worthwhile(const khrr_ctx* ctx, int n);
 int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const float* m, float* acc,
@@ -203,10 +207,10 @@ int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const flo
 // Batched pair-symmetric launches (i_per_lane 8) over a device item table
 // holding n_big items of 4096-body superblocks then n_small of 2048, then
 // the per-system slot reduction; max_n bounds every system.
-int sym_items_launch(khrr_ctx* ctx, int n_big, int n_small, const SymItem* items, int n_sys,
-                     const SymSys* sys, int max_n, int ld, const float* pos, const float* h,
-                     const float* m, const int* sb_flags, float* part, float* acc, float* phi,
-                     unsigned flags);
+int sym_items_launch(khrr_ctx* ctx, int n_big, int n_mid, int n_small, const SymItem* items,
+                     int n_sys, const SymSys* sys, int max_n, int ld, const float* pos,
+                     const float* h, const float* m, const int* sb_flags, float* part, float* acc,
+                     float* phi, unsigned flags);
 
 }  // namespace khrr
 

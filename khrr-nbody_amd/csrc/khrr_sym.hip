@@ -230,8 +230,8 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_kernel(int n, int n_sb,
 
 // Batched form: workgroup b takes items[b], an (A, B) item of one system of
 // a segment table (planes of stride ld); every item of one launch has the
-// superblock size 64 * 8 * W.
-template <int W, bool DO_ACC, bool DO_PHI, bool COMP>
+// superblock size 64 * IPL * W.
+template <int IPL, int W, bool DO_ACC, bool DO_PHI, bool COMP>
 __global__ __launch_bounds__(khrr::kWave* W) void sym_items_kernel(
     const khrr::SymItem* __restrict__ items, int ld, const float* __restrict__ pos,
     const float* __restrict__ m, const float* __restrict__ h, const int* __restrict__ flags,
@@ -239,9 +239,9 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_items_kernel(
   __shared__ float red[W][4][khrr::kWave];
   const khrr::SymItem it = items[blockIdx.x];
   const size_t b = (size_t)it.base;
-  sym_block<8, W, DO_ACC, DO_PHI, COMP>(it.n, it.A, it.B, pos + b, pos + ld + b,
-                                        pos + 2 * (size_t)ld + b, m + b, h + b,
-                                        flags + it.flag_base, part + it.part, red);
+  sym_block<IPL, W, DO_ACC, DO_PHI, COMP>(it.n, it.A, it.B, pos + b, pos + ld + b,
+                                          pos + 2 * (size_t)ld + b, m + b, h + b,
+                                          flags + it.flag_base, part + it.part, red);
 }
 
 // acc[i] (+)= sum_X part[X][0..2][i];  phi[i] (-)= sum_X part[X][3][i]
@@ -331,27 +331,32 @@ SymLauncher pick_sym_mode(bool a, bool p, bool c) {
   return c ? launch_sym<IPL, W, false, true, true> : launch_sym<IPL, W, false, true, false>;
 }
 
+// ipl 8 with 4096- or 2048-body superblocks, ipl 4 with 4096, 2048 or 1024
+// (1024-body superblocks always run with ipl 4: khrr::sym_ipl).
 SymLauncher pick_sym(int ipl, int sb, bool a, bool p, bool c) {
+  if (sb == 1024) return pick_sym_mode<4, 4>(a, p, c);
   if (ipl == 8) return sb == 4096 ? pick_sym_mode<8, 8>(a, p, c) : pick_sym_mode<8, 4>(a, p, c);
   return sb == 4096 ? pick_sym_mode<4, 16>(a, p, c) : pick_sym_mode<4, 8>(a, p, c);
 }
 
-template <int W, bool A, bool P, bool C>
+template <int IPL, int W, bool A, bool P, bool C>
 void launch_sym_items(hipStream_t st, int n_items, const khrr::SymItem* items, int ld,
                       const float* pos, const float* m, const float* h, const int* flags,
                       float* part) {
-  hipLaunchKernelGGL((sym_items_kernel<W, A, P, C>), dim3(n_items), dim3(khrr::kWave * W), 0, st,
-                     items, ld, pos, m, h, flags, part);
+  hipLaunchKernelGGL((sym_items_kernel<IPL, W, A, P, C>), dim3(n_items), dim3(khrr::kWave * W), 0,
+                     st, items, ld, pos, m, h, flags, part);
 }
 
 using SymItemsLauncher = void (*)(hipStream_t, int, const khrr::SymItem*, int, const float*,
                                   const float*, const float*, const int*, float*);
 
-template <int W>
+template <int IPL, int W>
 SymItemsLauncher pick_sym_items_mode(bool a, bool p, bool c) {
-  if (a && p) return c ? launch_sym_items<W, true, true, true> : launch_sym_items<W, true, true, false>;
-  if (a) return c ? launch_sym_items<W, true, false, true> : launch_sym_items<W, true, false, false>;
-  return c ? launch_sym_items<W, false, true, true> : launch_sym_items<W, false, true, false>;
+  if (a && p)
+    return c ? launch_sym_items<IPL, W, true, true, true> : launch_sym_items<IPL, W, true, true, false>;
+  if (a)
+    return c ? launch_sym_items<IPL, W, true, false, true> : launch_sym_items<IPL, W, true, false, false>;
+  return c ? launch_sym_items<IPL, W, false, true, true> : launch_sym_items<IPL, W, false, true, false>;
 }
 
 }  // namespace
@@ -366,8 +371,11 @@ constexpr size_t kSymPartBudget = size_t(40) << 30;
 // 4-wave workgroups per CU instead of two 8-wave ones), so a lone system
 // below a few 10^5 bodies leaves a smaller share of the launch to its last,
 // partly filled wave of workgroups; from 524288 bodies on the 4096 form has
-// items enough and half the partial-slot memory.
-int sym_superblock(int n) { return n >= 524288 ? 4096 : 2048; }
+// items enough and half the partial-slot memory.  Below 32768 bodies,
+// 1024-body superblocks (4 i per lane, 4 waves) keep 16 or more of them.
+int sym_superblock(int n) { return n >= 524288 ? 4096 : n >= 32768 ? 2048 : 1024; }
+
+int sym_ipl(int n, int ipl) { return sym_superblock(n) == 1024 ? 4 : ipl; }
 
 bool sym_eligible(int n, int ipl) {
   if (ipl != 4 && ipl != 8) return false;
@@ -391,6 +399,7 @@ int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const flo
   const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
   const int sb = sym_superblock(n);
   const int n_sb = (n + sb - 1) / sb;
+  ipl = sym_ipl(n, ipl);
   const size_t part_elems = (size_t)n_sb * 4 * n;
   void* base = nullptr;
   KHRR_TRY(scratch(ctx, Carver::need(n_sb, sizeof(int)) + Carver::need(part_elems, sizeof(float)),
@@ -412,20 +421,23 @@ int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const flo
   return check_launch(ctx, "sym_kernel");
 }
 
-int sym_items_launch(khrr_ctx* ctx, int n_big, int n_small, const SymItem* items, int n_sys,
-                     const SymSys* sys, int max_n, int ld, const float* pos, const float* h,
-                     const float* m, const int* sb_flags, float* part, float* acc, float* phi,
-                     unsigned flags) {
-  if (n_big + n_small == 0) return 0;
+int sym_items_launch(khrr_ctx* ctx, int n_big, int n_mid, int n_small, const SymItem* items,
+                     int n_sys, const SymSys* sys, int max_n, int ld, const float* pos,
+                     const float* h, const float* m, const int* sb_flags, float* part, float* acc,
+                     float* phi, unsigned flags) {
+  if (n_big + n_mid + n_small == 0) return 0;
   const bool do_acc = acc != nullptr, do_phi = phi != nullptr;
   const bool comp = (flags & KHRR_COMPENSATED) != 0;
   const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
-  if (n_big > 0)
-    pick_sym_items_mode<8>(do_acc, do_phi, comp)(ctx->stream, n_big, items, ld, pos, m, h, sb_flags,
-                                                 part);
-  if (n_small > 0)
-    pick_sym_items_mode<4>(do_acc, do_phi, comp)(ctx->stream, n_small, items + n_big, ld, pos, m, h,
-                                                 sb_flags, part);
+  if (n_big > 0)  // 4096-body superblocks
+    pick_sym_items_mode<8, 8>(do_acc, do_phi, comp)(ctx->stream, n_big, items, ld, pos, m, h,
+                                                    sb_flags, part);
+  if (n_mid > 0)  // 2048
+    pick_sym_items_mode<8, 4>(do_acc, do_phi, comp)(ctx->stream, n_mid, items + n_big, ld, pos, m,
+                                                    h, sb_flags, part);
+  if (n_small > 0)  // 1024
+    pick_sym_items_mode<4, 4>(do_acc, do_phi, comp)(ctx->stream, n_small, items + n_big + n_mid, ld,
+                                                    pos, m, h, sb_flags, part);
   const dim3 grid(ceil_div(max_n, 256), n_sys);
   if (comp)
     hipLaunchKernelGGL(sym_reduce_sys<true>, grid, dim3(256), 0, ctx->stream, sys, ld, (int)do_acc,

@@ -1,7 +1,7 @@
 """The batched form (khrr_accel_batched): many independent systems of uneven N
 in one persistent launch.  Each system's result must equal the plain
 single-system launch bit for bit, whatever else is in the batch: for
-n >= 32768 the pair-symmetric launch (symmetric 1, i_per_lane 8), below it
+n >= 16384 the pair-symmetric launch (symmetric 1, i_per_lane 8), below it
 the one-sided launch with j_split min(32, ceil(n/1024)), j_tile 256.  Results
 must also match the FP64 oracle within the gravity tolerance (median <= 1e-6,
 p99 <= 1e-5)."""
@@ -42,7 +42,7 @@ def _run_batched(ctx, off, pos, m, h, tune=None, phi=False, compensated=False):
 
 def _plain_tune(n):
     import khrr
-    if n >= 32768:
+    if n >= 16384:
         return khrr.Tune(8, 4, 1, 256, 1)
     return khrr.Tune(4, 4, max(1, min(32, -(-n // 1024))), 256, 0)
 
@@ -122,8 +122,8 @@ def test_composition_change_and_empty_systems(ctx):
 
 @pytest.mark.parametrize("compensated", [False, True])
 def test_large_systems_equal_plain(ctx, compensated):
-    # n < 32768: j-range cut into ~1024-body pieces summed in piece order;
-    # n >= 32768: pair-symmetric superblock items; both agree exactly with
+    # n < 16384: j-range cut into ~1024-body pieces summed in piece order;
+    # n >= 16384: pair-symmetric superblock items; both agree exactly with
     # the plain launch of the same shape
     sizes = [40000, 1000, 16385, 70000, 16384, 32768, 32767]
     off, pos, m, h = _population(sizes, seed0=700)

@@ -48,7 +48,7 @@ def _run_all(srv, reqs, arrivals):
 @pytest.mark.parametrize("kind", ["galaxy", "large"])
 def test_alone_equals_in_the_mix(ctx, kind):
     import khrr
-    # "large" runs on the pair-symmetric path (n >= 32768), "galaxy" on the
+    # "large" runs on the pair-symmetric path (n >= 16384), "galaxy" on the
     # one-sided one; the others span both
     if kind == "galaxy":
         target = dict(arrays=_galaxy_req(1024, 2048, 5), steps=12)

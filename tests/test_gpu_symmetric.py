@@ -94,8 +94,8 @@ def test_deterministic_and_close_to_one_sided(ctx):
 
 
 def test_small_systems_fall_back(ctx):
-    # below two superblocks the one-sided kernel runs
-    pos, _, mass, h = galaxy_system(1000, 2000, seed=6)
+    # below two superblocks (of 1024 bodies) the one-sided kernel runs
+    pos, _, mass, h = galaxy_system(300, 700, seed=6)
     p32, m32, h32 = pos.astype(np.float32), mass.astype(np.float32), h.astype(np.float32)
     acc, _ = _run(ctx, p32, m32, h32, _sym(), phi=False)
     ref, _ = orc.accel(p32, h32, p32, m32, h32, want_phi=False)
