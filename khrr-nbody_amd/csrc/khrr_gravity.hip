@@ -218,9 +218,12 @@ int validate_tune(khrr_ctx* ctx, const khrr_tune& t) {
   return 0;
 }
 
-// Static launch shape: the pair-symmetric kernel (4 i per lane) where it
+// Static launch shape: the pair-symmetric kernel (8 i per lane) where it
 // applies; otherwise 2 i per lane, 4 waves per block, 256-j tiles and a
-// j-split that gives ~8 waves per SIMD (4 SIMDs per CU).
+// j-split that gives ~8 waves per SIMD (4 SIMDs per CU), with j-ranges of
+// at least 1024 bodies, or one 256-body tile up to 8192 bodies (a small
+// system's few i-groups then run many short waves instead of a few long
+// ones).
 khrr_tune default_tune(const khrr_ctx* ctx, int n_i, int n_j) {
   khrr_tune t;
   t.i_per_lane = 2;
@@ -230,7 +233,8 @@ khrr_tune default_tune(const khrr_ctx* ctx, int n_i, int n_j) {
   const long long groups = (n_i + 64ll * t.i_per_lane - 1) / (64ll * t.i_per_lane);
   const long long target = 32ll * ctx->cu_count;
   long long split = groups > 0 ? (target + groups - 1) / groups : 1;
-  const long long max_split = n_j / 1024 > 1 ? n_j / 1024 : 1;
+  const long long min_chunk = n_j <= 8192 ? 256 : 1024;
+  const long long max_split = n_j / min_chunk > 1 ? n_j / min_chunk : 1;
   if (split > max_split) split = max_split;
   if (split < 1) split = 1;
   t.j_split = (int)split;
