@@ -61,6 +61,10 @@ def parse():
     ap.add_argument("--batched-steps", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-lambda", type=float, default=3.0, help="arrivals per second")
+    ap.add_argument("--serve-tmax", default="0.05,0.5",
+                    help="range of the requests' integration times tmax (uniform), lo,hi")
+    ap.add_argument("--serve-saturated", type=float, default=50.0,
+                    help="arrival rate of the saturated serving run (0: skip it)")
     ap.add_argument("--legs", default="binning,stencil,batched,serving,precision",
                     help="extras to run after the headline (comma list)")
     ap.add_argument("--precision-steps", type=int, default=3,
@@ -349,14 +353,14 @@ def poisson_arrivals(n_req, lam, seed=8):
 
 
 def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4e10,
-                  lam_saturated=50.0):
+                  lam_saturated=50.0, tmax=(0.05, 0.5)):
     """Continuous batching on a Poisson arrival stream vs the same requests
     run one at a time (FIFO) with plain per-system launches; then the same
     requests arriving faster than either can serve them (lam_saturated)."""
     import numpy as np
     import torch
     import khrr
-    reqs = serving_requests(n_req)
+    reqs = serving_requests(n_req, tmax_lo=tmax[0], tmax_hi=tmax[1])
     arrivals = poisson_arrivals(n_req, lam)
     grid = khrr.cube_grid(128, -2.0, 2.0)
     cells = 128 ** 3
@@ -393,6 +397,8 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4
                 k = ids[fid]
                 lat[k] = time.perf_counter() - t0 - arrivals[k]
                 srv.result(fid, cells)
+                if n_req >= 50 and len(lat) % 25 == 0:
+                    log(f"served {len(lat)}/{n_req}")
             st = srv.stats()
             if st["active"] == 0 and st["pending"] == 0:
                 busy += time.perf_counter() - busy_since
@@ -423,7 +429,9 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4
     # plain: each request alone, one khrr_accel per force evaluation
     dev = torch.device("cuda:0")
     service = []
-    for r in reqs:
+    for ri, r in enumerate(reqs):
+        if n_req >= 50 and ri % 25 == 0:
+            log(f"plain {ri}/{n_req}")
         torch.cuda.synchronize()
         ts = time.perf_counter()
         sim = Sim(ctx, r["pos"], r["vel"], r["mass"], 0.5 * r["eps"] ** 2, 0.002)
@@ -450,6 +458,21 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4
     plain_lat = fifo_latency(arrivals)
     log(f"plain FIFO: busy {service.sum():.2f} s, p99 {np.percentile(plain_lat, 99):.3f} s")
     head = policies["least_work_first"]
+    out = {"requests": n_req, "lambda_per_s": lam, "tmax_range": list(tmax), "dt": 0.002,
+           "grid": "128^3 over [-2,2]^3", "budget_GB": budget_gb, "max_active": max_active,
+           "mix": "90% Plummer N in [1024,131072] log-uniform, 10% galaxies 65536+131072",
+           "pair_interactions": pairs,
+           "policy": "least_work_first",
+           "served_busy_s": head["busy_s"], "served_Gpairs_per_s": head["Gpairs_per_s"],
+           "p50_latency_s": head["p50_latency_s"], "p99_latency_s": head["p99_latency_s"],
+           "policies": policies,
+           "plain_busy_s": float(service.sum()),
+           "plain_Gpairs_per_s": pairs / service.sum() / 1e9,
+           "plain_p50_latency_s": float(np.percentile(plain_lat, 50)),
+           "plain_p99_latency_s": float(np.percentile(plain_lat, 99)),
+           "throughput_vs_plain": float(service.sum() / head["busy_s"])}
+    if lam_saturated <= 0:
+        return out
     # saturated: every request queued within a fraction of a second
     arr_sat = poisson_arrivals(n_req, lam_saturated)
     lat_sat, fig_sat = served(round_pairs, arr_sat)
@@ -464,20 +487,8 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4
     log(f"served (saturated, lambda {lam_saturated}/s): busy {fig_sat['busy_s']:.2f} s, "
         f"p50 {fig_sat['p50_latency_s']:.3f} s, p99 {fig_sat['p99_latency_s']:.3f} s vs FIFO "
         f"p50 {saturated['plain_p50_latency_s']:.3f} s, p99 {saturated['plain_p99_latency_s']:.3f} s")
-    return {"requests": n_req, "lambda_per_s": lam, "tmax_range": [0.05, 0.5], "dt": 0.002,
-            "grid": "128^3 over [-2,2]^3", "budget_GB": budget_gb, "max_active": max_active,
-            "mix": "90% Plummer N in [1024,131072] log-uniform, 10% galaxies 65536+131072",
-            "pair_interactions": pairs,
-            "policy": "least_work_first",
-            "served_busy_s": head["busy_s"], "served_Gpairs_per_s": head["Gpairs_per_s"],
-            "p50_latency_s": head["p50_latency_s"], "p99_latency_s": head["p99_latency_s"],
-            "policies": policies,
-            "plain_busy_s": float(service.sum()),
-            "plain_Gpairs_per_s": pairs / service.sum() / 1e9,
-            "plain_p50_latency_s": float(np.percentile(plain_lat, 50)),
-            "plain_p99_latency_s": float(np.percentile(plain_lat, 99)),
-            "throughput_vs_plain": float(service.sum() / head["busy_s"]),
-            "saturated": saturated}
+    out["saturated"] = saturated
+    return out
 
 
 def cpu_baseline(sim, acc_gpu, rows_n):
@@ -651,7 +662,9 @@ def main():
         torch.cuda.empty_cache()
     if "serving" in legs and args.serve_requests > 0:
         log("serving")
-        result["serving"] = bench_serving(ctx, args.serve_requests, args.serve_lambda)
+        tmax = tuple(float(v) for v in args.serve_tmax.split(","))
+        result["serving"] = bench_serving(ctx, args.serve_requests, args.serve_lambda,
+                                          lam_saturated=args.serve_saturated, tmax=tmax)
         torch.cuda.empty_cache()
     if "precision" in legs and args.precision_steps > 0:
         log("precision form")
