@@ -92,3 +92,34 @@ def test_invalid_grids_rejected():
 def test_empty_input():
     out = orc.p2mesh_bin([-1] * 3, [0.5] * 3, [4] * 3, np.zeros((3, 0)), np.zeros(0))
     assert out["count"].sum() == 0 and np.all(out["offset"] == 0)
+
+
+def test_cell_phi_pins():
+    # computeCellPhiUnsorted (p2mesh-kernel.cuh:152-167): every particle adds
+    # phi_i / count[cell] to its cell, so a cell holds the mean potential of
+    # its particles.  Pinned by what that fixes: a one-particle cell gets its
+    # phi exactly, a cell of equal phi and a power-of-two count gets that
+    # value exactly, count-weighted cell values sum to sum(phi), empty cells
+    # stay 0 (or keep their value when accumulating), and accumulating adds
+    rng = np.random.default_rng(12)
+    G = 40
+    cell_id = np.concatenate([
+        np.full(4, 3), np.full(8, 7), np.array([11]), np.array([19]),
+        rng.integers(20, 36, 500)]).astype(np.int32)
+    phi = rng.normal(size=cell_id.shape[0])
+    phi[:4] = -1.75   # cell 3: four equal values
+    phi[4:12] = 0.3   # cell 7: eight equal values
+    count = np.bincount(cell_id, minlength=G).astype(np.int32)
+    out = orc.p2mesh_cell_phi(cell_id, count, phi)
+    assert out[3] == -1.75
+    assert out[7] == 0.3 * 8 / 8 or abs(out[7] - 0.3) <= 4e-16
+    assert out[11] == phi[12] and out[19] == phi[13]
+    empty = count == 0
+    assert np.all(out[empty] == 0.0)
+    np.testing.assert_allclose(np.sum(out * count), np.sum(phi), rtol=1e-12)
+    for c in range(20, 36):  # mean per cell
+        np.testing.assert_allclose(out[c], np.mean(phi[cell_id == c]), rtol=1e-13, atol=1e-15)
+    start = rng.normal(size=G)
+    acc = orc.p2mesh_cell_phi(cell_id, count, phi, cell_phi=start, accumulate=True)
+    np.testing.assert_allclose(acc, start + out, rtol=1e-14, atol=1e-15)
+    assert np.all(acc[empty] == start[empty])
