@@ -12,8 +12,6 @@ Also reported on the same line (one GPU):
   roofline      gravity kernel: 20 FLOP per pair vs the 157.3 TFLOP/s FP32
                 peak, average launch time from HIP events on the library's
                 stream; traffic from rocprof counters when supplied
-  accuracy      FP32 force vs the FP64 oracle on 16384 sampled bodies
-  cpu_baseline  the FP64 oracle (OpenMP, all host cores) on those rows
   binning       p2mesh of the final state on 256^3 over [-2,2]^3
   stencil       200 damped-wave iterations on that grid from phi = 0
   batched       512 Plummer spheres (N log-uniform 1024..131072) per step,
@@ -21,9 +19,14 @@ Also reported on the same line (one GPU):
   serving       a Poisson stream of mixed requests through the server
                 (two round schedules) vs the same requests one at a time
   precision     3.1M-body two-galaxy encounter: plain vs compensated sums,
-                force error vs the FP64 oracle and drift over a few steps
+                time per evaluation and drift over a few steps
   drift         E, P, L relative drift over the timed steps (FP64 device
                 reductions)
+  cpu leg       last, unless --no-cpu: the FP64 oracle (OpenMP, all host
+                cores) timed on 16384 rows of the headline workload
+                (cpu_baseline), and as the checker of the GPU legs the FP64
+                references behind `accuracy` (headline forces), the
+                stencil's err_vs_fp64 and the precision form's errors
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -128,29 +131,25 @@ def energy(d):
     return 0.5 * (d["Ek"] + d["Ep"])
 
 
-def bench_precision(ctx, steps, samples=8192):
+def bench_precision(ctx, steps, check, samples=8192):
     """Two-galaxy encounter (2 x (524288 + 1048576) = 3145728 bodies): plain
-    FP32 sums vs compensated (TwoSum) sums.  Force error against the FP64
-    oracle on `samples` random bodies at t = 0, and FP64 energy, momentum and
-    angular-momentum drift over `steps` predictor-corrector steps."""
+    FP32 sums vs compensated (TwoSum) sums: time per force evaluation and
+    FP64 energy, momentum and angular-momentum drift over `steps`
+    predictor-corrector steps.  The forces of `samples` random bodies at
+    t = 0 go to `check` for the CPU leg's FP64 comparison."""
     import numpy as np
     import torch
     import ics
     import khrr
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as orc
     pos, vel, mass, h = ics.encounter()
     n = pos.shape[1]
     rows = np.sort(np.random.default_rng(samples).choice(n, samples, replace=False))
-    m32 = mass.astype(np.float32).astype(np.float64)
-    h32 = h.astype(np.float32).astype(np.float64)
-    p32 = pos.astype(np.float32).astype(np.float64)
-    t0 = time.perf_counter()
-    ref, _ = orc.accel_rows(rows, p32, h32, m32)
-    t_orc = time.perf_counter() - t0
+    check.update(rows=rows, pos=pos.astype(np.float32).astype(np.float64),
+                 m=mass.astype(np.float32).astype(np.float64),
+                 h=h.astype(np.float32).astype(np.float64), acc={})
     out = {"workload": "two-galaxy encounter, 2 x (524288 + 1048576) bodies, X = +-2, "
                        "tilts 0/45 deg, V = (-+0.3, +-0.05)",
-           "n_bodies": n, "samples": samples, "steps": steps, "oracle_s": t_orc}
+           "n_bodies": n, "samples": samples, "steps": steps}
     for mode in ("plain", "compensated"):
         sim = Sim(ctx, pos, vel, mass, h, 0.002)
         sim.compensated = mode == "compensated"
@@ -160,8 +159,7 @@ def bench_precision(ctx, steps, samples=8192):
         sim.init_forces()
         torch.cuda.synchronize()
         launches, ms = ctx.kernel_time(khrr.K_GRAVITY)
-        acc0 = sim.acc.cpu().numpy()[:, rows].astype(np.float64)
-        e = np.linalg.norm(acc0 - ref, axis=0) / np.linalg.norm(ref, axis=0)
+        check["acc"][mode] = sim.acc.cpu().numpy()[:, rows].astype(np.float64)
         d0 = sim.diagnostics()
         for k in range(steps):
             sim.step()
@@ -174,13 +172,10 @@ def bench_precision(ctx, steps, samples=8192):
         out[mode] = {
             "force_ms": ms / max(launches, 1),
             "Gpairs_per_s": float(n) * n / (ms / max(launches, 1) * 1e-3) / 1e9,
-            "err_median": float(np.median(e)), "err_p99": float(np.percentile(e, 99)),
-            "err_max": float(np.max(e)),
             "dE_rel": energy(d1) / energy(d0) - 1.0,
             "dP_abs": math.sqrt(sum((d1[k] - d0[k]) ** 2 for k in ("Px", "Py", "Pz"))),
             "dLz_rel": d1["Lz"] / d0["Lz"] - 1.0}
         log(f"precision {mode}: {out[mode]['force_ms']:.1f} ms per force evaluation, "
-            f"error median {out[mode]['err_median']:.2e} p99 {out[mode]['err_p99']:.2e}, "
             f"dE/E {out[mode]['dE_rel']:.2e} over {steps} steps")
         del sim
         torch.cuda.empty_cache()
@@ -218,7 +213,10 @@ def bench_binning(ctx, sim, reps=5):
         "occupied_cells": int((bufs["count"] > 0).sum().item())}
 
 
-def bench_stencil(ctx, cell_mass, iters=200):
+def bench_stencil(ctx, cell_mass, check, iters=200):
+    """200 damped-wave iterations on 256^3 from phi = 0; the mass grid and
+    the final field go to `check` for the CPU leg's FP64 comparison."""
+    import numpy as np
     import torch
     import khrr
     nx = 256
@@ -237,6 +235,8 @@ def bench_stencil(ctx, cell_mass, iters=200):
     _, ms = ctx.kernel_time(khrr.K_WAVE)
     avg_ms = ms / iters
     alg = 16 * G
+    check.update(nx=nx, iters=iters, cell_mass=cell_mass.cpu().numpy(),
+                 phi=cur.cpu().numpy().astype(np.float64))
     return {"grid": "256^3", "iterations": iters, "ms_per_iter": avg_ms, "alg_bytes_per_iter": alg,
             "alg_GBps": alg / (avg_ms * 1e-3) / 1e9,
             "frac_hbm_peak": alg / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
@@ -491,27 +491,28 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4
     return out
 
 
-def cpu_baseline(sim, acc_gpu, rows_n):
-    """FP64 oracle on a bounded sample of i-rows at the final positions."""
+def cpu_leg(checks, rows_n, result):
+    """The CPU leg: the FP64 oracle (OpenMP, all host cores) timed on a
+    bounded sample of the headline workload -- the reported CPU baseline --
+    and, as the checker of the GPU legs, the FP64 references behind their
+    error figures: headline forces at the final positions, the stencil
+    field after 200 iterations, the precision form's forces at t = 0."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    p32 = sim.pos32.cpu().numpy().astype(np.float64)
-    m = sim.m32.cpu().numpy().astype(np.float64)
-    h = sim.h32.cpu().numpy().astype(np.float64)
-    rows = np.sort(np.random.default_rng(4096).choice(sim.n, rows_n, replace=False))
+    hd = checks["headline"]
+    n = hd["pos"].shape[1]
+    rows = np.sort(np.random.default_rng(4096).choice(n, rows_n, replace=False))
     t0 = time.perf_counter()
-    ref, _ = orc.accel_rows(rows, p32, h, m)
+    ref, _ = orc.accel_rows(rows, hd["pos"], hd["h"], hd["m"])
     dt = time.perf_counter() - t0
-    got = acc_gpu[:, rows].astype(np.float64)
-    e = np.linalg.norm(got - ref, axis=0) / np.linalg.norm(ref, axis=0)
-    acc = {"rows": int(rows_n), "median": float(np.median(e)),
-           "p99": float(np.percentile(e, 99)), "max": float(np.max(e)),
-           "reference": "FP64 oracle on the same FP32 inputs"}
-    pairs = rows_n * sim.n
-    cpu = {"value": pairs / dt / 1e9, "unit": "G pair-interactions/s",
+    e = np.linalg.norm(hd["acc"][:, rows] - ref, axis=0) / np.linalg.norm(ref, axis=0)
+    result["accuracy"] = {"rows": int(rows_n), "median": float(np.median(e)),
+                          "p99": float(np.percentile(e, 99)), "max": float(np.max(e)),
+                          "reference": "FP64 oracle on the same FP32 inputs"}
+    cpu = {"value": rows_n * n / dt / 1e9, "unit": "G pair-interactions/s",
            "cores": orc.num_threads(), "kind": "port",
-           "sample": f"{rows_n} random i-rows x {sim.n} j (one force evaluation subsample, "
+           "sample": f"{rows_n} random i-rows x {n} j (one force evaluation subsample, "
                      f"FP64, OpenMP), {dt:.2f} s"}
     try:
         with open("/proc/cpuinfo") as f:
@@ -520,7 +521,27 @@ def cpu_baseline(sim, acc_gpu, rows_n):
         cpu["cpu_model"] = model
     except OSError:
         pass
-    return cpu, acc
+    result["cpu_baseline"] = cpu
+    st = checks.get("stencil")
+    if st and "stencil" in result:
+        log("cpu leg: FP64 stencil reference")
+        nx = st["nx"]
+        ref_phi, _ = orc.wave_iterate(orc.reference_wave(nx, 4.0 / nx), np.zeros(nx ** 3),
+                                      np.zeros(nx ** 3), st["cell_mass"], st["iters"])
+        result["stencil"]["err_vs_fp64"] = float(np.max(np.abs(st["phi"] - ref_phi)) /
+                                                 np.max(np.abs(ref_phi)))
+        result["stencil"]["err_definition"] = "max|phi32 - phi64| / max|phi64| after all iterations"
+    pr = checks.get("precision")
+    if pr and "precision" in result:
+        log("cpu leg: FP64 forces of the precision-form sample")
+        ref, _ = orc.accel_rows(pr["rows"], pr["pos"], pr["h"], pr["m"])
+        for mode, acc in pr["acc"].items():
+            e = np.linalg.norm(acc - ref, axis=0) / np.linalg.norm(ref, axis=0)
+            result["precision"][mode].update(err_median=float(np.median(e)),
+                                             err_p99=float(np.percentile(e, 99)),
+                                             err_max=float(np.max(e)))
+            log(f"precision {mode}: error median {np.median(e):.2e} "
+                f"p99 {np.percentile(e, 99):.2e}")
 
 
 def main():
@@ -634,16 +655,18 @@ def main():
         "integrator_ms_per_launch": int_ms / max(int_launches, 1),
         "drift": drift,
     }
-    if not args.no_cpu or not args.quick:
+    # what the CPU leg checks, collected by the GPU legs (host copies)
+    checks = {}
+    if not args.no_cpu:
         # fresh force at the final positions for the accuracy sample
         chk = torch.zeros((3, n), device="cuda:0")
         sim.gravity(chk)
         torch.cuda.synchronize()
-        if not args.no_cpu:
-            log(f"cpu baseline: oracle on {args.cpu_rows} rows")
-            cpu, acc = cpu_baseline(sim, chk.cpu().numpy(), args.cpu_rows)
-            result["cpu_baseline"] = cpu
-            result["accuracy"] = acc
+        checks["headline"] = dict(pos=sim.pos32.cpu().numpy().astype(np.float64),
+                                  m=sim.m32.cpu().numpy().astype(np.float64),
+                                  h=sim.h32.cpu().numpy().astype(np.float64),
+                                  acc=chk.cpu().numpy().astype(np.float64))
+        del chk
     legs = set() if args.quick else set(args.legs.split(","))
     if legs:
         ctx.set_timing(True)
@@ -653,7 +676,8 @@ def main():
         result["binning"] = binning
         if "stencil" in legs:
             log("stencil")
-            result["stencil"] = bench_stencil(ctx, cell_mass)
+            checks["stencil"] = {}
+            result["stencil"] = bench_stencil(ctx, cell_mass, checks["stencil"])
         del cell_mass
         torch.cuda.empty_cache()
     if "batched" in legs:
@@ -668,7 +692,11 @@ def main():
         torch.cuda.empty_cache()
     if "precision" in legs and args.precision_steps > 0:
         log("precision form")
-        result["precision"] = bench_precision(ctx, args.precision_steps)
+        checks["precision"] = {}
+        result["precision"] = bench_precision(ctx, args.precision_steps, checks["precision"])
+    if not args.no_cpu:
+        log(f"cpu leg: oracle on {args.cpu_rows} rows of the headline workload")
+        cpu_leg(checks, args.cpu_rows, result)
     print(json.dumps(result), flush=True)
 
 
