@@ -18,9 +18,10 @@
 //              Two binary searches on the sorted cells give the number of
 //              particles before the block (= the reference's block-granular
 //              start_id) and through it (= maxPBC); an LDS histogram and
-//              scan give per-cell counts and exact exclusive offsets.  Each
-//              non-empty cell walks its sorted segment in index order to
-//              write id_in_cell and the FP64 mass sum.
+//              scan give per-cell counts and exact exclusive offsets;
+//              id_in_cell is the sorted position minus the cell's offset,
+//              and each cell adds its particles' FP64 masses in index order
+//              from chunks of the block's masses gathered into LDS.
 // Every cell is written exactly once, so no output needs clearing first.
 #include <algorithm>
 #include <climits>
@@ -203,16 +204,24 @@ __global__ __launch_bounds__(256) void block_bounds(int n, int nb, const int* __
 }
 
 // One workgroup per 512-cell block of the reference.  Empty blocks (most
-// of a fine grid) are four coalesced stores per cell; in the others each
-// cell walks its sorted segment in index order, eight particles per trip
-// (their indices and masses are loaded before the in-order FP64 adds).
+// of a fine grid) are four coalesced stores per cell.  In the others an
+// LDS histogram and scan give counts and exact offsets; id_in_cell of the
+// particle at sorted position p is p minus its cell's offset (the sort
+// keeps index order within a cell); the block's masses are gathered into
+// LDS in chunks by all threads and each cell adds its own in index order,
+// so the FP64 sum is that of a sequential walk while the loads run in
+// parallel (a cell of many particles costs a few chunk round trips instead
+// of two dependent loads per eight particles).
 __global__ __launch_bounds__(kCellBlock) void bin_cells(
     int n, long long G, const int* __restrict__ skeys, const int* __restrict__ svals,
     const int* __restrict__ bounds, const double* __restrict__ mass, int* __restrict__ count,
     int* __restrict__ offset, int* __restrict__ start_ref, int* __restrict__ block_incl,
     int* __restrict__ id_in_cell, double* __restrict__ cell_mass) {
+  constexpr int kMassChunk = 2048;
   __shared__ int cnt[kCellBlock];
+  __shared__ int exs[kCellBlock];
   __shared__ int wsum[kCellBlock / khrr::kWave];
+  __shared__ double msh[kMassChunk];
   const int tid = threadIdx.x;
   const long long c0 = (long long)blockIdx.x * kCellBlock;
   const long long c = c0 + tid;
@@ -246,28 +255,26 @@ __global__ __launch_bounds__(kCellBlock) void bin_cells(
   int before = 0;
   for (int w = 0; w < wave; ++w) before += wsum[w];
   const int excl = lo + before + incl - mine;
-  if (c >= G) return;
-  count[c] = mine;
-  if (offset) offset[c] = excl;
-  if (start_ref) start_ref[c] = lo;
-  double m = 0.0;
-  constexpr int kB = 8;
-  for (int k0 = 0; k0 < mine; k0 += kB) {
-    int v[kB];
-    double mv[kB];
-#pragma unroll
-    for (int j = 0; j < kB; ++j) v[j] = k0 + j < mine ? svals[excl + k0 + j] : 0;
-#pragma unroll
-    for (int j = 0; j < kB; ++j) mv[j] = (cell_mass && k0 + j < mine) ? mass[v[j]] : 0.0;
-#pragma unroll
-    for (int j = 0; j < kB; ++j) {
-      if (k0 + j < mine) {
-        if (id_in_cell) id_in_cell[v[j]] = k0 + j;
-        m += mv[j];
-      }
-    }
+  exs[tid] = excl;
+  if (c < G) {
+    count[c] = mine;
+    if (offset) offset[c] = excl;
+    if (start_ref) start_ref[c] = lo;
   }
-  if (cell_mass) cell_mass[c] = m;
+  __syncthreads();
+  if (id_in_cell)
+    for (int p = lo + tid; p < hi; p += kCellBlock) id_in_cell[svals[p]] = p - exs[skeys[p] - (int)c0];
+  if (!cell_mass) return;  // the same for the whole workgroup
+  double m = 0.0;
+  for (int p0 = lo; p0 < hi; p0 += kMassChunk) {
+    const int p1 = min(hi, p0 + kMassChunk);
+    for (int p = p0 + tid; p < p1; p += kCellBlock) msh[p - p0] = mass[svals[p]];
+    __syncthreads();
+    const int q1 = min(excl + mine, p1);
+    for (int q = max(excl, p0); q < q1; ++q) m += msh[q - p0];
+    __syncthreads();  // msh is refilled by the next chunk
+  }
+  if (c < G) cell_mass[c] = m;
 }
 
 // perm[offset[cell_id[i]] + id_in_cell[i]] = i: the sorted segments back
