@@ -90,6 +90,22 @@ def test_all_in_one_cell_and_empty(ctx):
     assert np.all(empty["cell_mass"] == 0.0)
 
 
+def test_dense_cells_across_mass_chunks(ctx):
+    # three crowded cells of one 512-cell block, interleaved in index order,
+    # with masses over many magnitudes: the block's masses are gathered in
+    # chunks of 2048 whose boundaries cut through the cells, and each cell's
+    # FP64 sum must still be the index-order one
+    rng = np.random.default_rng(17)
+    g = _grid(4, -1.0, 1.0)  # 64 cells, one block
+    centres = np.array([[-0.75, -0.75, -0.75], [0.25, -0.25, 0.75], [0.75, 0.75, -0.25]])
+    which = rng.choice(3, 7200, p=[0.35, 0.4, 0.25])
+    pos = centres[which].T + rng.uniform(-0.2, 0.2, (3, which.size))
+    mass = 10.0 ** rng.uniform(-12, 3, which.size)
+    got = _bin_gpu(ctx, g, pos, mass)
+    _check_equal(got, _bin_ref(g, pos, mass))
+    assert np.sort(got["count"])[-3:].min() > 1500
+
+
 def test_reference_start_file_fixture(ctx):
     import os
     from conftest import GOLDEN

@@ -97,7 +97,7 @@ def test_zero_stays_zero_and_rotation(ctx):
 
 
 @pytest.mark.parametrize("nx,iters", [(37, 5), (70, 4), (20, 2), (64, 7), (18, 6), (130, 9),
-                                     (33, 12), (96, 10), (5, 8)])
+                                     (33, 12), (96, 10), (5, 8), (1, 4), (2, 5), (3, 6)])
 def test_two_step_pass_equals_single_steps(ctx, nx, iters):
     # khrr_wave_iterate runs pairs of iterations as one two-step pass; the
     # fields and the buffer roles must equal single steps with the
