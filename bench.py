@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--quick", action="store_true", help="headline only (no extras)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=2)
+    ap.add_argument("--batched-run-steps", type=int, default=1000,
+                    help="steps of the batched-run leg (512 systems advanced together)")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-lambda", type=float, default=3.0, help="arrivals per second")
     ap.add_argument("--serve-tmax", default="0.05,0.5",
@@ -69,7 +71,8 @@ def parse():
     ap.add_argument("--serve-saturated", type=float, default=50.0,
                     help="arrival rate of the saturated serving run (0: skip it)")
     ap.add_argument("--legs", default="binning,stencil,batched,serving,precision",
-                    help="extras to run after the headline (comma list)")
+                    help="extras to run after the headline (comma list; also batched-run, "
+                         "the 512 systems advanced --batched-run-steps steps)")
     ap.add_argument("--precision-steps", type=int, default=3,
                     help="steps of the 3.1M-body encounter per sum mode (0: skip)")
     ap.add_argument("--shape", default=None,
@@ -319,6 +322,50 @@ def bench_batched(ctx, steps):
             "speedup_vs_plain_reference_shape": t_r / t_b,
             "plain_reference_shape": "khrr_accel per system with one body per lane and no "
                                      "j split (the reference's one-thread-per-body launch)"}
+
+
+def bench_batched_run(ctx, steps, budget_gb=32.0):
+    """The 512 Plummer spheres of the batched leg advanced `steps`
+    predictor-corrector steps together: all submitted at once to one server,
+    so every round is one segmented predictor launch, one batched gravity
+    evaluation (a replayed graph) and one segmented corrector launch over
+    all systems.  Reports the rate and each system's FP64 conservation."""
+    import numpy as np
+    import ics
+    import khrr
+    sizes, eps = ics.plummer_population(512, 1024, 131072, seed=7)
+    grid = khrr.cube_grid(16, -2.0, 2.0)
+    srv = khrr.Server(ctx, int(budget_gb * 1e9), len(sizes))
+    ids = []
+    for k, n in enumerate(sizes):
+        p, v = ics.plummer(int(n), 1.0, 0.3333, 1.0, seed=100000 + k)
+        ids.append(srv.submit(p, v, np.full(n, 1.0 / n), np.full(n, float(eps[k])), 0.002, steps,
+                              grid))
+    t0 = time.perf_counter()
+    done = []
+    while len(done) < len(ids):
+        done += srv.step()
+        st = srv.stats()
+        if st["rounds"] % 100 == 0 and st["rounds"] > 0 and not done:
+            log(f"batched run: round {st['rounds']}/{steps}")
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    dE, dP, dL = [], [], []
+    for rid in ids:
+        d0, d1, _ = srv.result(rid, 16 ** 3)
+        dE.append(abs(energy(d1) / energy(d0) - 1.0))
+        dP.append(math.sqrt(sum((d1[k] - d0[k]) ** 2 for k in ("Px", "Py", "Pz"))))
+        dL.append(math.sqrt(sum((d1[k] - d0[k]) ** 2 for k in ("Lx", "Ly", "Lz"))))
+    srv.close()
+    pairs = float(np.sum(sizes.astype(np.float64) ** 2)) * (steps + 3)
+    dE, dP, dL = np.array(dE), np.array(dP), np.array(dL)
+    return {"systems": len(sizes), "bodies": int(np.sum(sizes)), "steps": steps, "dt": 0.002,
+            "rounds": st["rounds"], "wall_s": wall, "ms_per_step": wall / steps * 1e3,
+            "Gpairs_per_s": pairs / wall / 1e9,
+            "dE_rel_median": float(np.median(dE)), "dE_rel_max": float(np.max(dE)),
+            "dP_abs_max": float(np.max(dP)), "dL_abs_max": float(np.max(dL)),
+            "note": "per-system FP64 diagnostics at t = 0 and after the last step; "
+                    "mass 1 per system, so dP and dL are absolute"}
 
 
 def serving_requests(n_req, seed=7, tmax_lo=0.05, tmax_hi=0.5, dt=0.002):
@@ -683,6 +730,10 @@ def main():
     if "batched" in legs:
         log("batched")
         result["batched"] = bench_batched(ctx, args.batched_steps)
+        torch.cuda.empty_cache()
+    if "batched-run" in legs:
+        log(f"batched run: 512 systems, {args.batched_run_steps} steps")
+        result["batched_run"] = bench_batched_run(ctx, args.batched_run_steps)
         torch.cuda.empty_cache()
     if "serving" in legs and args.serve_requests > 0:
         log("serving")
