@@ -488,6 +488,11 @@ class Server:
         return rid.value
 
     def step(self, max_done: int = 4096):
+        """One scheduling round; returns the ids that finished in it.  A
+        round that admits or finishes a simulation returns when the device
+        has completed it; others return once queued (at most two in
+        flight), so time the server from the step that finds work to the
+        step that drains it."""
         ids = (ctypes.c_longlong * max_done)()
         nd = ctypes.c_int()
         self._ctx._check(self._lib.khrr_server_step(self._h, ids, max_done, ctypes.byref(nd)))
