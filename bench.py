@@ -288,6 +288,14 @@ def bench_batched(ctx, steps):
         for p, hh, mm, o in per_sys:
             ctx.accel(p, hh, p, mm, hh, acc=o)
 
+    # the reference's one-sided method with the library's static launch shape
+    one_sided = [ctx.tune_default(int(n)) for n in sizes]
+
+    def grav_one_sided(out):
+        del out
+        for (p, hh, mm, o), t in zip(per_sys, one_sided):
+            ctx.accel(p, hh, p, mm, hh, acc=o, tune=t)
+
     # the reference's launch shape: one body per lane, no j split
     ref_shape = khrr.Tune(1, 4, 1, 256, 0)
 
@@ -310,6 +318,9 @@ def bench_batched(ctx, steps):
     grav_plain(sim.acct)
     t_p = run(grav_plain, max(1, steps // 2))
     log(f"plain per-system: {t_p * 1e3:.1f} ms per force evaluation")
+    grav_one_sided(sim.acct)
+    t_o = run(grav_one_sided, 1)
+    log(f"plain per-system, one-sided: {t_o * 1e3:.1f} ms per force evaluation")
     grav_ref_shape(sim.acct)
     t_r = run(grav_ref_shape, 1)
     log(f"plain per-system, reference launch shape: {t_r * 1e3:.1f} ms per force evaluation")
@@ -318,6 +329,10 @@ def bench_batched(ctx, steps):
             "batched_Gpairs_per_s": pairs / t_b / 1e9, "plain_Gpairs_per_s": pairs / t_p / 1e9,
             "speedup_vs_plain": t_p / t_b, "tune": tune.as_dict(), "graph": True,
             "plain": "khrr_accel per system, default launch shape, one stream",
+            "plain_one_sided_ms": t_o * 1e3,
+            "speedup_vs_plain_one_sided": t_o / t_b,
+            "plain_one_sided": "khrr_accel per system with the static one-sided shape (every "
+                               "directed pair evaluated, the reference's method)",
             "plain_reference_shape_ms": t_r * 1e3,
             "speedup_vs_plain_reference_shape": t_r / t_b,
             "plain_reference_shape": "khrr_accel per system with one body per lane and no "
