@@ -255,6 +255,21 @@ int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* p
 int khrr_p2mesh_cell_phi(khrr_ctx* ctx, int n, long long n_cells, const int* cell_id,
                          const int* id_in_cell, const int* count, const int* offset,
                          const float* phi, double* cell_phi, int accumulate);
+/* apply_particle_to_cell_weighted (p2mesh-kernel.cuh:273-287): for every
+ * particle i of cell c, cell_value[c] += value[i] * (mass[i] / cell_mass[c]),
+ * summed in FP64 in particle index order from 0 (the reference adds with
+ * FP64 atomics into a fresh buffer; nbody.cu:545,585).  cell_mass is the
+ * binning's FP64 cell mass; cells of no bodies get 0.  Same binning inputs
+ * as khrr_p2mesh_cell_phi.  value, mass: n doubles; cell_value: G doubles. */
+int khrr_p2mesh_cell_weighted(khrr_ctx* ctx, int n, long long n_cells, const int* cell_id,
+                              const int* id_in_cell, const int* count, const int* offset,
+                              const double* cell_mass, const double* mass, const double* value,
+                              double* cell_value);
+/* apply_cell_to_particles_avg (p2mesh-kernel.cuh:256-271): out[i] =
+ * cell_value[c] / count[c] for the particle's cell c, -1 where count[c] is
+ * 0, NaN where c is outside [0, n_cells).  out: n doubles. */
+int khrr_p2mesh_cell_avg(khrr_ctx* ctx, int n, long long n_cells, const int* cell_id,
+                         const int* count, const double* cell_value, double* out);
 
 /* ---- damped-wave Poisson stencil (a8) -------------------------------- */
 typedef struct {
@@ -290,6 +305,16 @@ int khrr_wave_iterate(khrr_ctx* ctx, const khrr_wave* w, float** prev, float** c
  * every cell gets its own 3-D stencil.  nx >= 2. */
 int khrr_mesh_acc(khrr_ctx* ctx, int nx, double dx, const float* cell_phi, float* cell_acc,
                   float* cell_acc_abs);
+/* The same field of an FP64 potential, in FP64 throughout: the form the
+ * debug dumps of convert_particles apply to the FP64 cell potential of the
+ * bodies (nbody.cu:484-517).  cell_acc: vec3 planes of nx^3 doubles;
+ * cell_acc_abs: nx^3 doubles or NULL. */
+int khrr_mesh_acc_f64(khrr_ctx* ctx, int nx, double dx, const double* cell_phi,
+                      double* cell_acc, double* cell_acc_abs);
+/* acc_abs (p2mesh-kernel.cuh:289-298): out[i] = |v_i| of FP32 vec3 planes,
+ * evaluated in FP64 as sqrt((x*x + y*y) + z*z) (norm3, common.cuh:84-96).
+ * v: vec3 planes of n floats; out: n doubles. */
+int khrr_acc_abs(khrr_ctx* ctx, int n, const float* v, double* out);
 /* apply_acceleration_to_particles + apply_cell_to_particles
  * (p2mesh-kernel.cuh:226-252): acc[.][i] = cell_acc[.][cell_id[i]] and
  * phi[i] = cell_phi[cell_id[i]] (cell_id from khrr_p2mesh_bin; an index

@@ -4,7 +4,10 @@
 // Replaces acc_field (nbody/include/p2mesh-kernel.cuh:169-224),
 // apply_acceleration_to_particles (:226-238) and apply_cell_to_particles
 // (:240-252), called by calc_acceleration_by_grid (nbody/src/nbody.cu:243-262)
-// after the wave iterations.
+// after the wave iterations, and acc_abs (:289-298).  The FP64 form of the
+// field and acc_abs serve the debug dumps of convert_particles
+// (nbody.cu:287-701), which difference the FP64 cell potential of the
+// bodies and take |a| of body and cell accelerations.
 //
 // acc_field indexes cells in 3-D but the reference launches it over a 1-D
 // grid of cells (nbody.cu:244, over_cells x over_blocks), so there every
@@ -16,7 +19,8 @@
 //
 // Arithmetic: the FP32 field values are widened to FP64, the reference's
 // expression is evaluated in FP64 (the difference of two floats is exact
-// there), and the result is rounded to FP32 once.  HBM traffic per cell:
+// there), and the result is rounded to FP32 once; the FP64 form is the
+// same expression on FP64 values.  HBM traffic per cell:
 // read phi (4 B, neighbours from cache), write 3 x 4 B (+4 B for |a|).
 #include <cmath>
 
@@ -28,17 +32,20 @@ namespace {
 
 constexpr int kMX = 64, kMY = 4, kMZ = 32;
 
-__device__ __forceinline__ double diff1(bool lo_face, bool hi_face, float minus, float c,
-                                        float plus) {
-  if (lo_face) return (double)plus - (double)c;
-  if (hi_face) return (double)c - (double)minus;
-  return 0.5 * ((double)plus - (double)minus);
+__device__ __forceinline__ double diff1(bool lo_face, bool hi_face, double minus, double c,
+                                        double plus) {
+  if (lo_face) return plus - c;
+  if (hi_face) return c - minus;
+  return 0.5 * (plus - minus);
 }
 
+// T = float: the field is widened to FP64 on load and the results rounded
+// to FP32 once; T = double: plain FP64.
+template <typename T>
 __global__ __launch_bounds__(kMX* kMY) void mesh_acc_kernel(int nx, double coef,
-                                                            const float* __restrict__ phi,
-                                                            float* __restrict__ acc,
-                                                            float* __restrict__ acc_abs) {
+                                                            const T* __restrict__ phi,
+                                                            T* __restrict__ acc,
+                                                            T* __restrict__ acc_abs) {
   const int ix = blockIdx.x * kMX + threadIdx.x;
   const int iy = blockIdx.y * kMY + threadIdx.y;
   if (ix >= nx || iy >= nx) return;
@@ -47,25 +54,38 @@ __global__ __launch_bounds__(kMX* kMY) void mesh_acc_kernel(int nx, double coef,
   const size_t plane = (size_t)nx * nx;
   const size_t G = plane * nx;
   size_t at = (size_t)ix + (size_t)iy * nx + (size_t)z0 * plane;
-  float cm = z0 > 0 ? phi[at - plane] : 0.f;
-  float cc = phi[at];
+  double cm = z0 > 0 ? (double)phi[at - plane] : 0.0;
+  double cc = phi[at];
   for (int z = z0; z < z1; ++z, at += plane) {
-    const float cp = z + 1 < nx ? phi[at + plane] : 0.f;
-    const float xm = ix > 0 ? phi[at - 1] : 0.f, xp = ix + 1 < nx ? phi[at + 1] : 0.f;
-    const float ym = iy > 0 ? phi[at - nx] : 0.f, yp = iy + 1 < nx ? phi[at + nx] : 0.f;
+    const double cp = z + 1 < nx ? (double)phi[at + plane] : 0.0;
+    const double xm = ix > 0 ? (double)phi[at - 1] : 0.0;
+    const double xp = ix + 1 < nx ? (double)phi[at + 1] : 0.0;
+    const double ym = iy > 0 ? (double)phi[at - nx] : 0.0;
+    const double yp = iy + 1 < nx ? (double)phi[at + nx] : 0.0;
     double dx = diff1(ix == 0, ix == nx - 1, xm, cc, xp);
     double dy = diff1(iy == 0, iy == nx - 1, ym, cc, yp);
     double dz = diff1(z == 0, z == nx - 1, cm, cc, cp);
     dx *= coef;
     dy *= coef;
     dz *= coef;
-    acc[at] = (float)dx;
-    acc[G + at] = (float)dy;
-    acc[2 * G + at] = (float)dz;
-    if (acc_abs) acc_abs[at] = (float)sqrt(dx * dx + dy * dy + dz * dz);
+    acc[at] = (T)dx;
+    acc[G + at] = (T)dy;
+    acc[2 * G + at] = (T)dz;
+    if (acc_abs) acc_abs[at] = (T)sqrt(dx * dx + dy * dy + dz * dz);
     cm = cc;
     cc = cp;
   }
+}
+
+// acc_abs (p2mesh-kernel.cuh:289-298): |v| of FP32 vec3 planes, in FP64
+// (norm3, common.cuh:84-96: (x*x + y*y) + z*z, then sqrt).
+__global__ __launch_bounds__(256) void acc_abs_kernel(int n, const float* __restrict__ v,
+                                                      double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t N = (size_t)n;
+  const double x = v[i], y = v[N + i], z = v[2 * N + i];
+  out[i] = sqrt(x * x + y * y + z * z);
 }
 
 // acc[c][i] = cell_acc[c][cell_id[i]], phi[i] = cell_phi[cell_id[i]]; an
@@ -92,10 +112,9 @@ __global__ __launch_bounds__(256) void mesh_gather_kernel(int n, long long G,
 
 }  // namespace
 
-extern "C" {
-
-int khrr_mesh_acc(khrr_ctx* ctx, int nx, double dx, const float* cell_phi, float* cell_acc,
-                  float* cell_acc_abs) {
+template <typename T>
+static int mesh_acc(khrr_ctx* ctx, int nx, double dx, const T* cell_phi, T* cell_acc,
+                    T* cell_acc_abs) {
   KHRR_TRY(khrr::enter(ctx));
   if (nx < 2 || (long long)nx * nx * nx > (1ll << 31))
     return khrr::fail(ctx, KHRR_EINVAL, "nx must be in [2, 1290] (got %d)", nx);
@@ -105,9 +124,32 @@ int khrr_mesh_acc(khrr_ctx* ctx, int nx, double dx, const float* cell_phi, float
   const dim3 block(kMX, kMY);
   const dim3 grid(khrr::ceil_div(nx, kMX), khrr::ceil_div(nx, kMY), khrr::ceil_div(nx, kMZ));
   khrr::Timed tm(ctx, KHRR_K_WAVE);
-  hipLaunchKernelGGL(mesh_acc_kernel, grid, block, 0, ctx->stream, nx, coef, cell_phi, cell_acc,
-                     cell_acc_abs);
+  hipLaunchKernelGGL(mesh_acc_kernel<T>, grid, block, 0, ctx->stream, nx, coef, cell_phi,
+                     cell_acc, cell_acc_abs);
   return khrr::check_launch(ctx, "mesh_acc_kernel");
+}
+
+extern "C" {
+
+int khrr_mesh_acc(khrr_ctx* ctx, int nx, double dx, const float* cell_phi, float* cell_acc,
+                  float* cell_acc_abs) {
+  return mesh_acc<float>(ctx, nx, dx, cell_phi, cell_acc, cell_acc_abs);
+}
+
+int khrr_mesh_acc_f64(khrr_ctx* ctx, int nx, double dx, const double* cell_phi,
+                      double* cell_acc, double* cell_acc_abs) {
+  return mesh_acc<double>(ctx, nx, dx, cell_phi, cell_acc, cell_acc_abs);
+}
+
+int khrr_acc_abs(khrr_ctx* ctx, int n, const float* v, double* out) {
+  KHRR_TRY(khrr::enter(ctx));
+  if (n < 0) return khrr::fail(ctx, KHRR_EINVAL, "bad size n=%d", n);
+  if (n > 0 && (!v || !out)) return khrr::fail(ctx, KHRR_EINVAL, "null pointer");
+  if (n == 0) return 0;
+  khrr::Timed tm(ctx, KHRR_K_BIN);
+  hipLaunchKernelGGL(acc_abs_kernel, dim3(khrr::ceil_div(n, 256)), dim3(256), 0, ctx->stream, n,
+                     v, out);
+  return khrr::check_launch(ctx, "acc_abs_kernel");
 }
 
 int khrr_mesh_to_particles(khrr_ctx* ctx, int n, long long n_cells, const int* cell_id,

@@ -314,6 +314,51 @@ __global__ __launch_bounds__(256) void cell_phi_kernel(int n, long long G,
   cell_phi[c] = v;
 }
 
+// apply_particle_to_cell_weighted (p2mesh-kernel.cuh:273-287) in index
+// order: sum of value[i] * (mass[i] / cell_mass[c]) over the cell's bodies.
+__global__ __launch_bounds__(256) void cell_weighted_kernel(int n, long long G,
+                                                            const int* __restrict__ count,
+                                                            const int* __restrict__ offset,
+                                                            const int* __restrict__ perm,
+                                                            const double* __restrict__ cell_mass,
+                                                            const double* __restrict__ mass,
+                                                            const double* __restrict__ value,
+                                                            double* __restrict__ cell_value) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G) return;
+  const int k = count[c];
+  double v = 0.0;
+  if (k > 0) {
+    const double cm = cell_mass[c];
+    const long long o = offset[c];
+    for (int q = 0; q < k; ++q) {
+      if (o + q < 0 || o + q >= n) break;
+      const int i = perm[o + q];
+      if (i >= 0 && i < n) v += value[i] * (mass[i] / cm);
+    }
+  }
+  cell_value[c] = v;
+}
+
+// apply_cell_to_particles_avg (p2mesh-kernel.cuh:256-271): the cell's value
+// shared out over its bodies, -1 for a cell of no bodies, NaN for an index
+// outside the grid.
+__global__ __launch_bounds__(256) void cell_avg_kernel(int n, long long G,
+                                                       const int* __restrict__ cell_id,
+                                                       const int* __restrict__ count,
+                                                       const double* __restrict__ cell_value,
+                                                       double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell_id[i];
+  if (c < 0 || c >= G) {
+    out[i] = __builtin_nan("");
+    return;
+  }
+  const int k = count[c];
+  out[i] = k > 0 ? cell_value[c] / k : -1.0;
+}
+
 }  // namespace
 
 extern "C" int khrr_p2mesh_cell_phi(khrr_ctx* ctx, int n, long long n_cells, const int* cell_id,
@@ -335,6 +380,46 @@ extern "C" int khrr_p2mesh_cell_phi(khrr_ctx* ctx, int n, long long n_cells, con
   hipLaunchKernelGGL(cell_phi_kernel, dim3(ceil_div(n_cells, 256)), dim3(256), 0, ctx->stream, n,
                      n_cells, count, offset, perm, phi, cell_phi, accumulate);
   return check_launch(ctx, "cell_phi kernels");
+}
+
+extern "C" int khrr_p2mesh_cell_weighted(khrr_ctx* ctx, int n, long long n_cells,
+                                         const int* cell_id, const int* id_in_cell,
+                                         const int* count, const int* offset,
+                                         const double* cell_mass, const double* mass,
+                                         const double* value, double* cell_value) {
+  using namespace khrr;
+  KHRR_TRY(enter(ctx));
+  if (n < 0 || n_cells < 1 || n_cells > INT_MAX)
+    return fail(ctx, KHRR_EINVAL, "bad sizes n=%d cells=%lld", n, n_cells);
+  if (!count || !offset || !cell_mass || !cell_value ||
+      (n > 0 && (!cell_id || !id_in_cell || !mass || !value)))
+    return fail(ctx, KHRR_EINVAL, "null pointer");
+  void* base = nullptr;
+  KHRR_TRY(scratch(ctx, Carver::need((size_t)(n > 0 ? n : 1), sizeof(int)), &base));
+  int* perm = static_cast<int*>(base);
+  Timed tm(ctx, KHRR_K_BIN);
+  if (n > 0)
+    hipLaunchKernelGGL(segments_from_bins, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, n,
+                       n_cells, cell_id, id_in_cell, offset, perm);
+  hipLaunchKernelGGL(cell_weighted_kernel, dim3(ceil_div(n_cells, 256)), dim3(256), 0,
+                     ctx->stream, n, n_cells, count, offset, perm, cell_mass, mass, value,
+                     cell_value);
+  return check_launch(ctx, "cell_weighted kernels");
+}
+
+extern "C" int khrr_p2mesh_cell_avg(khrr_ctx* ctx, int n, long long n_cells, const int* cell_id,
+                                    const int* count, const double* cell_value, double* out) {
+  using namespace khrr;
+  KHRR_TRY(enter(ctx));
+  if (n < 0 || n_cells < 1 || n_cells > INT_MAX)
+    return fail(ctx, KHRR_EINVAL, "bad sizes n=%d cells=%lld", n, n_cells);
+  if (n > 0 && (!cell_id || !count || !cell_value || !out))
+    return fail(ctx, KHRR_EINVAL, "null pointer");
+  if (n == 0) return 0;
+  Timed tm(ctx, KHRR_K_BIN);
+  hipLaunchKernelGGL(cell_avg_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, n,
+                     n_cells, cell_id, count, cell_value, out);
+  return check_launch(ctx, "cell_avg_kernel");
 }
 
 extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos,

@@ -5,6 +5,7 @@
 #include <charconv>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <sys/stat.h>
 
 namespace khrr_host {
@@ -309,13 +310,11 @@ bool write_sample(const std::string& out_dir, int it, int it_all, double t, cons
   return true;
 }
 
-bool write_cell_phi(const std::string& out_dir, int iter, int nx, double sim_l, double dx,
-                    const std::vector<double>& cell_phi) {
-  const std::string dir = join(out_dir, "phi");
-  ::mkdir(dir.c_str(), 0755);
-  File fp(join(dir, "phi_" + width5(iter) + ".csv"), "w");
+bool write_x_slice(const std::string& path, const char* header, int nx, double sim_l, double dx,
+                   const std::vector<double>& cell_value, int decimals) {
+  File fp(path, "w");
   if (!fp.f) return false;
-  std::fprintf(fp.f, "x, phi\n");
+  std::fprintf(fp.f, "%s\n", header);
   const int iy = nx / 2, iz = nx / 2;
   for (int i = 0; i < nx; ++i) {
     const double x = -sim_l + i * dx;
@@ -323,7 +322,50 @@ bool write_cell_phi(const std::string& out_dir, int iter, int nx, double sim_l, 
     auto res = std::to_chars(xs, xs + sizeof(xs) - 1, x);  // std::format "{}": shortest form
     *res.ptr = '\0';
     const size_t at = (size_t)i + (size_t)iy * nx + (size_t)iz * nx * nx;
-    std::fprintf(fp.f, "%s, %.7f\n", xs, cell_phi[at]);
+    std::fprintf(fp.f, "%s, %.*f\n", xs, decimals, cell_value[at]);
+  }
+  return true;
+}
+
+bool write_cell_phi(const std::string& out_dir, int iter, int nx, double sim_l, double dx,
+                    const std::vector<double>& cell_phi) {
+  const std::string dir = join(out_dir, "phi");
+  ::mkdir(dir.c_str(), 0755);
+  return write_x_slice(join(dir, "phi_" + width5(iter) + ".csv"), "x, phi", nx, sim_l, dx,
+                       cell_phi, 7);
+}
+
+bool write_index_list(const std::string& path, const std::vector<double>& values) {
+  File fp(path, "w");
+  if (!fp.f) return false;
+  std::fprintf(fp.f, "i, acc\n");
+  for (size_t i = 0; i < values.size(); ++i) std::fprintf(fp.f, "%zu, %.7f\n", i, values[i]);
+  return true;
+}
+
+bool write_count_slice(const std::string& path, int nx, const std::vector<int>& count) {
+  File fp(path, "w");
+  if (!fp.f) return false;
+  std::fprintf(fp.f, "xi, counts\n");
+  const size_t row = (size_t)(nx / 2) * nx + (size_t)(nx / 2) * nx * nx;
+  for (int i = 0; i < nx; ++i)
+    std::fprintf(fp.f, "%d, %s\n", i, ostream_double((double)count[row + i]).c_str());
+  return true;
+}
+
+bool write_count_distribution(const std::string& dir, const std::vector<int>& count) {
+  std::map<int, long long> dist;
+  for (int c : count) ++dist[c];
+  File fp(join(dir, "particles_in_cell.txt"), "w");
+  File fs(join(dir, "particles_in_cell_partial_sum.txt"), "w");
+  if (!fp.f || !fs.f) return false;
+  std::fprintf(fp.f, "n_particles, cells_with_so_many_particles\n");
+  double partial = 0.;
+  size_t k = 0;
+  for (auto it = dist.rbegin(); it != dist.rend(); ++it, ++k) {
+    std::fprintf(fp.f, "%d, %lld\n", it->first, it->second);
+    partial += (double)it->first * (double)it->second;
+    if (k > 0) std::fprintf(fs.f, "%zu, %s\n", k, ostream_double(partial).c_str());
   }
   return true;
 }

@@ -90,10 +90,25 @@ class ResultWriter {
 bool write_sample(const std::string& out_dir, int it, int it_all, double t, const double pos[3],
                   const double vel[3], const double acc[3], double phi);
 
-// out/phi/phi_{:5}.csv: x-slice of the cell potential at iy = iz = NX/2
-// (print_cell_phi, nbody.cu:704-724).
+// "x, <value>" rows along the x line of cells at iy = iz = NX/2, x = -sim_l +
+// i*dx in std::format's shortest "{}" form and the value with `decimals`
+// fixed decimals (print_cell_phi nbody.cu:704-724, the slices of
+// convert_particles :428-645 and check_mass :107-112).
+bool write_x_slice(const std::string& path, const char* header, int nx, double sim_l, double dx,
+                   const std::vector<double>& cell_value, int decimals);
+// out/phi/phi_{:5}.csv: x-slice of the cell potential (print_cell_phi).
 bool write_cell_phi(const std::string& out_dir, int iter, int nx, double sim_l, double dx,
                     const std::vector<double>& cell_phi);
+
+// Debug dumps of convert_particles (nbody.cu:287-701) into data/nbody/debug:
+// "i, acc" rows of one value per body, seven decimals (:622-697).
+bool write_index_list(const std::string& path, const std::vector<double>& values);
+// particle_counts.txt: "xi, counts" along the x line (:412-432).
+bool write_count_slice(const std::string& path, int nx, const std::vector<int>& count);
+// particles_in_cell.txt (count, cells with that count, largest count first)
+// and particles_in_cell_partial_sum.txt (running total of bodies over those
+// rows from the second on) of check_particles_distribution (:68-96).
+bool write_count_distribution(const std::string& dir, const std::vector<int>& count);
 
 // %g-style text of a double as std::ostream's default float format prints it.
 std::string ostream_double(double v);

@@ -23,6 +23,9 @@
 //   * the cell-potential buffer is accumulated across saves as in the
 //     reference (never cleared, :1628-1634); --clear-cell-phi clears it.
 //
+// --debug-dumps writes the files of convert_particles (nbody.cu:287-701)
+// to data/nbody/debug after the initial forces (debug_dumps below).
+//
 // --solver wave selects the reference's other force (NBodySolver::Wave,
 // :1128-1143, hard-wired off there): calc_acceleration_by_grid (:183-285)
 // bins the bodies, advances the damped-wave field (20000 iterations at
@@ -37,6 +40,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <sys/stat.h>
+#include <type_traits>
 #include <vector>
 
 #include "khrr.h"
@@ -57,13 +62,14 @@ struct Options {
   bool json = false;
   bool wave = false;
   int setup_iters = 20000;  // _iterations_to_setup, nbody.cu:1143
+  bool debug_dumps = false;
 };
 
 void usage() {
   std::fprintf(stderr,
                "usage: khrr_nbody [--data DIR] [--compat-accumulate] [--compensated]\n"
                "                  [--clear-cell-phi] [--max-steps N] [--grid NX DX] [--json]\n"
-               "                  [--solver nbody|wave] [--setup-iters N]\n");
+               "                  [--solver nbody|wave] [--setup-iters N] [--debug-dumps]\n");
 }
 
 bool parse(int argc, char** argv, Options& o) {
@@ -91,6 +97,8 @@ bool parse(int argc, char** argv, Options& o) {
         return false;
       }
       o.wave = v == "wave";
+    } else if (a == "--debug-dumps") {
+      o.debug_dumps = true;
     } else if (a == "--setup-iters" && i + 1 < argc) {
       o.setup_iters = std::atoi(argv[++i]);
       if (o.setup_iters < 0) return false;
@@ -180,6 +188,148 @@ bool load_restart(const std::string& bin, int i_cont, const std::vector<GalaxyPr
   std::fclose(f);
   if (!ok) err = "short LIE_0.bin";
   return ok;
+}
+
+// Debug dumps of convert_particles (nbody.cu:287-701, with the checks of
+// :35-127), written to data/nbody/debug at the place of its commented-out
+// call after the initial forces (:1488-1504), from the current state: the
+// FP64 positions and masses, the body potential phi and acceleration acc.
+// need_init_phi is true there: the binning feeds a wave field started from
+// rest, advanced `iters` iterations (the call names `_iterations`, which
+// run() does not define; --setup-iters gives the count).  The reference
+// sums with FP64 atomics; here every per-cell sum runs in body index order.
+// The wave field and the cell acceleration are FP32 (the stencil's format);
+// the bodies' cell potential and its gradient are FP64 as in the reference.
+int debug_dumps(khrr_ctx* ctx, const std::string& dir, int n, int nx, double dx, double sim_l,
+                const khrr_grid& grid, const khrr_wave& wave, int iters, const double* d_pos,
+                const double* d_mass, const float* d_phi, const float* d_acc) {
+  ::mkdir(dir.c_str(), 0755);
+  const size_t N = (size_t)n;
+  const long long G = (long long)nx * nx * nx;
+  DevBuffers db;
+  int* cell = db.alloc<int>(N);
+  int* idc = db.alloc<int>(N);
+  int* count = db.alloc<int>(G);
+  int* off = db.alloc<int>(G);
+  double* cmass = db.alloc<double>(G);
+  float* w[3] = {db.alloc<float>(G), db.alloc<float>(G), db.alloc<float>(G)};
+  float* src = db.alloc<float>(G);
+  float* cacc = db.alloc<float>(3 * G);
+  float* cabs = db.alloc<float>(G);
+  double* cphi_part = db.alloc<double>(G);
+  double* cacc64 = db.alloc<double>(3 * G);
+  double* cabs64 = db.alloc<double>(G);
+  double* cval = db.alloc<double>(G);
+  double* pabs = db.alloc<double>(N);
+  double* cpabs = db.alloc<double>(N);
+  float* pacc = db.alloc<float>(3 * N);
+  double* pavg = db.alloc<double>(N);
+  if (!pavg) {
+    std::fprintf(stderr, "Error: device allocation failed\n");
+    return 1;
+  }
+  auto path = [&](const char* name) { return dir + "/" + name; };
+  auto to_host = [&](const auto* d, size_t count_, auto& v) {
+    v.resize(count_);
+    return hipMemcpy(v.data(), d, count_ * sizeof(v[0]), hipMemcpyDeviceToHost) == hipSuccess;
+  };
+  auto cells_f64 = [&](const auto* d, std::vector<double>& v) {
+    using T = std::remove_cv_t<std::remove_pointer_t<decltype(d)>>;
+    std::vector<T> h;
+    if (!to_host(d, (size_t)G, h)) return false;
+    v.assign(h.begin(), h.end());
+    return true;
+  };
+  auto sum = [](const std::vector<double>& v) {
+    double s = 0.;
+    for (double x : v) s += x;
+    return s;
+  };
+  bool ok = true;
+  auto slice = [&](const char* name, const char* header, const std::vector<double>& v, int dec) {
+    ok = ok && write_x_slice(path(name), header, nx, sim_l, dx, v, dec);
+  };
+
+  // binning, wave field from rest, its acceleration (nbody.cu:333-380)
+  CHECK_K(ctx, khrr_p2mesh_bin(ctx, &grid, n, d_pos, d_mass, cell, idc, count, off, nullptr,
+                               nullptr, cmass));
+  CHECK_K(ctx, khrr_wave_source(ctx, &wave, cmass, src));
+  float *prev = w[0], *curr = w[1], *next = w[2];
+  CHECK_K(ctx, khrr_wave_iterate(ctx, &wave, &prev, &curr, &next, src, iters));
+  CHECK_K(ctx, khrr_mesh_acc(ctx, nx, dx, curr, cacc, cabs));
+
+  std::vector<double> cell_mass, cell_phi, pmass;
+  std::vector<int> cnt, cid;
+  if (!cells_f64(cmass, cell_mass) || !cells_f64(curr, cell_phi) || !to_host(count, G, cnt) ||
+      !to_host(cell, N, cid) || !to_host(d_mass, N, pmass)) {
+    std::fprintf(stderr, "Error: debug dump copy failed\n");
+    return 1;
+  }
+  // check_cell_particle_info_match (:48-66): host recount of the bodies
+  {
+    std::vector<int> manual(G, 0);
+    for (int c : cid)
+      if (c >= 0 && c < G) ++manual[c];
+    for (long long c = 0; c < G; ++c)
+      if (manual[c] != cnt[c])
+        std::printf("cell info mismatch!!! %d : %d\n", manual[c], cnt[c]);
+  }
+  // check_energy (:35-46)
+  double ep = 0.;
+  for (long long c = 0; c < G; ++c) ep += 0.5 * cell_mass[c] * cell_phi[c];
+  std::printf("ep: %g\n", ep);
+  // check_particles_distribution (:68-96), check_mass (:98-127)
+  ok = ok && write_count_distribution(dir, cnt);
+  slice("mass_cell.txt", "x, mass", cell_mass, 6);
+  std::printf("total mass in cells: %s\n", ostream_double(sum(cell_mass)).c_str());
+  std::printf("total mass in particles: %s\n", ostream_double(sum(pmass)).c_str());
+  // particle_counts, phi_cell (:412-446)
+  ok = ok && write_count_slice(path("particle_counts.txt"), nx, cnt);
+  slice("phi_cell.txt", "x, phi", cell_phi, 7);
+  // phi_part: cell mean of the bodies' potential (:449-472)
+  std::vector<double> h;
+  CHECK_K(ctx, khrr_p2mesh_cell_phi(ctx, n, G, cell, idc, count, off, d_phi, cphi_part, 0));
+  if (!cells_f64(cphi_part, h)) return 1;
+  slice("phi_part.txt", "x, phi", h, 7);
+  // acc_cell: |a| of the wave field (:475-488)
+  if (!cells_f64(cabs, h)) return 1;
+  slice("acc_cell.txt", "x, acc", h, 7);
+  // acc_part: |a| of the gradient of the bodies' cell potential (:491-526)
+  CHECK_K(ctx, khrr_mesh_acc_f64(ctx, nx, dx, cphi_part, cacc64, cabs64));
+  if (!cells_f64(cabs64, h)) return 1;
+  slice("acc_part.txt", "x, acc", h, 7);
+  // nbody_cell_acc: mass-weighted cell mean of the bodies' |a| (:529-558)
+  CHECK_K(ctx, khrr_acc_abs(ctx, n, d_acc, pabs));
+  CHECK_K(ctx, khrr_p2mesh_cell_weighted(ctx, n, G, cell, idc, count, off, cmass, d_mass, pabs,
+                                         cval));
+  if (!cells_f64(cval, h)) return 1;
+  slice("nbody_cell_acc.txt", "x, acc", h, 7);
+  // acc_cell_cycle: cell acceleration to the bodies and back (:561-598)
+  CHECK_K(ctx, khrr_mesh_to_particles(ctx, n, G, cell, cacc, nullptr, pacc, nullptr));
+  CHECK_K(ctx, khrr_acc_abs(ctx, n, pacc, cpabs));
+  CHECK_K(ctx, khrr_p2mesh_cell_weighted(ctx, n, G, cell, idc, count, off, cmass, d_mass, cpabs,
+                                         cval));
+  if (!cells_f64(cval, h)) return 1;
+  slice("acc_cell_cycle.txt", "x, acc", h, 7);
+  // cell_part_acc, nbody_part_acc (:602-645)
+  if (!to_host(cpabs, N, h)) return 1;
+  std::printf("cell part acc sum: %s\n", ostream_double(sum(h)).c_str());
+  ok = ok && write_index_list(path("cell_part_acc.txt"), h);
+  if (!to_host(pabs, N, h)) return 1;
+  std::printf("nbody part acc sum: %s\n", ostream_double(sum(h)).c_str());
+  ok = ok && write_index_list(path("nbody_part_acc.txt"), h);
+  // mpart_from_cell, mpart (:648-697)
+  CHECK_K(ctx, khrr_p2mesh_cell_avg(ctx, n, G, cell, count, cmass, pavg));
+  if (!to_host(pavg, N, h)) return 1;
+  std::printf("total mpart from cell: %s\n", ostream_double(sum(h)).c_str());
+  ok = ok && write_index_list(path("mpart_from_cell.txt"), h);
+  std::printf("total mpart: %s\n", ostream_double(sum(pmass)).c_str());
+  ok = ok && write_index_list(path("mpart.txt"), pmass);
+  if (!ok) {
+    std::fprintf(stderr, "Error: can't write the debug dumps in %s\n", dir.c_str());
+    return 1;
+  }
+  return 0;
 }
 
 int run(const Options& opt) {
@@ -383,6 +533,11 @@ int run(const Options& opt) {
   } else {
     CHECK_K(ctx, khrr_accel(ctx, n, d_pos32, d_h32, n, d_pos32, d_m32, d_h32, d_acc, d_phi, gflags,
                             nullptr));
+  }
+  if (opt.debug_dumps) {
+    if (debug_dumps(ctx, opt.data + "/debug", n, nx, opt.grid_dx, sim_l, grid, wave,
+                    opt.setup_iters, d_pos, d_mass, d_phi, d_acc) != 0)
+      return 1;
   }
   if (it == 1) {
     Diag d0;

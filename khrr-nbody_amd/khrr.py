@@ -119,6 +119,11 @@ _SIGS = {
     "khrr_p2mesh_cell_phi": (_I, [_P, _I, ctypes.c_longlong, _P, _P, _P, _P, _P, _P, _I]),
     "khrr_mesh_acc": (_I, [_P, _I, _D, _P, _P, _P]),
     "khrr_mesh_to_particles": (_I, [_P, _I, ctypes.c_longlong, _P, _P, _P, _P, _P]),
+    "khrr_p2mesh_cell_weighted": (_I, [_P, _I, ctypes.c_longlong, _P, _P, _P, _P, _P, _P, _P,
+                                       _P]),
+    "khrr_p2mesh_cell_avg": (_I, [_P, _I, ctypes.c_longlong, _P, _P, _P, _P]),
+    "khrr_mesh_acc_f64": (_I, [_P, _I, _D, _P, _P, _P]),
+    "khrr_acc_abs": (_I, [_P, _I, _P, _P]),
     "khrr_server_create": (_I, [_P, ctypes.c_size_t, _I, ctypes.POINTER(_P)]),
     "khrr_server_destroy": (None, [_P]),
     "khrr_server_set_round_pairs": (_I, [_P, _D]),
@@ -364,6 +369,33 @@ class Context:
             _ptr(offset, i32, (G,), "offset"), _ptr(phi, torch.float32, (n,), "phi"),
             _ptr(cell_phi, torch.float64, (G,), "cell_phi"), 1 if accumulate else 0))
 
+    def p2mesh_cell_weighted(self, cell_id, id_in_cell, count, offset, cell_mass, mass, value,
+                             cell_value):
+        """apply_particle_to_cell_weighted in index order: cell_value[c] =
+        sum over the cell's bodies of value[i] * (mass[i] / cell_mass[c])."""
+        import torch
+        n = cell_id.shape[0]
+        G = count.shape[0]
+        i32, f64 = torch.int32, torch.float64
+        self._check(self._lib.khrr_p2mesh_cell_weighted(
+            self._h, n, G, _ptr(cell_id, i32, (n,), "cell_id"),
+            _ptr(id_in_cell, i32, (n,), "id_in_cell"), _ptr(count, i32, (G,), "count"),
+            _ptr(offset, i32, (G,), "offset"), _ptr(cell_mass, f64, (G,), "cell_mass"),
+            _ptr(mass, f64, (n,), "mass"), _ptr(value, f64, (n,), "value"),
+            _ptr(cell_value, f64, (G,), "cell_value")))
+
+    def p2mesh_cell_avg(self, cell_id, count, cell_value, out):
+        """apply_cell_to_particles_avg: out[i] = cell_value[c] / count[c]
+        (-1 for an empty cell, NaN for an index outside the grid)."""
+        import torch
+        n = cell_id.shape[0]
+        G = count.shape[0]
+        self._check(self._lib.khrr_p2mesh_cell_avg(
+            self._h, n, G, _ptr(cell_id, torch.int32, (n,), "cell_id"),
+            _ptr(count, torch.int32, (G,), "count"),
+            _ptr(cell_value, torch.float64, (G,), "cell_value"),
+            _ptr(out, torch.float64, (n,), "out")))
+
     # -- stencil -------------------------------------------------------
     def wave_source(self, wave: Wave, cell_mass, src):
         import torch
@@ -406,6 +438,24 @@ class Context:
             self._h, int(nx), float(dx), _ptr(cell_phi, f32, (G,), "cell_phi"),
             _ptr(cell_acc, f32, (3, G), "cell_acc"),
             _ptr(cell_acc_abs, f32, (G,), "cell_acc_abs")))
+
+    def mesh_acc_f64(self, nx: int, dx: float, cell_phi, cell_acc, cell_acc_abs=None):
+        """acc_field in FP64: cell_acc (3, nx^3) float64 of cell_phi (nx^3 float64)."""
+        import torch
+        f64 = torch.float64
+        G = nx ** 3
+        self._check(self._lib.khrr_mesh_acc_f64(
+            self._h, int(nx), float(dx), _ptr(cell_phi, f64, (G,), "cell_phi"),
+            _ptr(cell_acc, f64, (3, G), "cell_acc"),
+            _ptr(cell_acc_abs, f64, (G,), "cell_acc_abs")))
+
+    def acc_abs(self, v, out):
+        """acc_abs: out[i] = |v[:, i]| in FP64 of float32 vec3 planes (3, n)."""
+        import torch
+        n = v.shape[-1]
+        self._check(self._lib.khrr_acc_abs(
+            self._h, n, _ptr(v, torch.float32, (3, n), "v"),
+            _ptr(out, torch.float64, (n,), "out")))
 
     def mesh_to_particles(self, cell_id, cell_acc=None, cell_phi=None, acc=None, phi=None):
         """acc[:, i] = cell_acc[:, cell_id[i]], phi[i] = cell_phi[cell_id[i]]."""

@@ -413,6 +413,38 @@ void orc_p2mesh_cell_phi(int n, long long G, const int* cell_id, const int* coun
   }
 }
 
+/* apply_particle_to_cell_weighted p2mesh-kernel.cuh:273-287: every body
+ * adds its value times its share of the cell's mass, weight =
+ * particle_mass / cell_mass, here in index order into a zeroed buffer. */
+void orc_p2mesh_cell_weighted(int n, long long G, const int* cell_id, const double* cell_mass,
+                              const double* mass, const double* value, double* cell_value) {
+  for (long long c = 0; c < G; ++c) cell_value[c] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const int c = cell_id[i];
+    const double weight = mass[i] / cell_mass[c];
+    cell_value[c] += value[i] * weight;
+  }
+}
+
+/* apply_cell_to_particles_avg p2mesh-kernel.cuh:256-271. */
+void orc_p2mesh_cell_avg(int n, const int* cell_id, const int* count, const double* cell_value,
+                         double* out) {
+  for (int i = 0; i < n; ++i) {
+    const int c = cell_id[i];
+    out[i] = count[c] > 0 ? cell_value[c] / count[c] : -1.;
+  }
+}
+
+/* acc_abs p2mesh-kernel.cuh:289-298: norm3 = sqrt(dot3(v, v)) with dot3
+ * summed x, y, z left to right (common.cuh:84-96). */
+void orc_acc_abs(int n, const double* v, double* out) {
+  const size_t N = (size_t)n;
+  for (int i = 0; i < n; ++i) {
+    const double x = v[i], y = v[N + i], z = v[2 * N + i];
+    out[i] = sqrt(x * x + y * y + z * z);
+  }
+}
+
 /* ---------------------------------------------------------------------- */
 /* acc_field p2mesh-kernel.cuh:169-224 with every cell at its own (ix, iy,
  * iz) (the 3-D stencil the kernel's indexing describes): per axis a

@@ -35,6 +35,13 @@
  *                      fields (exact inside, closed-form face error),
  *                      numpy.gradient with first-order edges.
  *   orc_mesh_to_particles  numpy fancy indexing.
+ *   orc_p2mesh_cell_weighted  constant value -> that value, single-body
+ *                      cells exact, conservation sum_c M_c v_c = sum_i m_i v_i,
+ *                      linearity, permutation of bodies inside a cell.
+ *   orc_p2mesh_cell_avg  equal masses -> the body mass exactly, -1 for an
+ *                      empty cell.
+ *   orc_acc_abs        Pythagorean quadruples exact, sign and axis
+ *                      invariance, numpy.linalg.norm.
  */
 #ifndef KHRR_ORACLE_H
 #define KHRR_ORACLE_H
@@ -93,6 +100,16 @@ int orc_p2mesh_bin(const double gmin[3], const double gcell[3], const int gsize[
 /* computeCellPhiUnsorted p2mesh-kernel.cuh:152-167 in index order. */
 void orc_p2mesh_cell_phi(int n, long long G, const int* cell_id, const int* count,
                          const double* phi, double* cell_phi, int accumulate);
+/* apply_particle_to_cell_weighted p2mesh-kernel.cuh:273-287 in index
+ * order from 0: cell_value[c] += value[i] * (mass[i] / cell_mass[c]). */
+void orc_p2mesh_cell_weighted(int n, long long G, const int* cell_id, const double* cell_mass,
+                              const double* mass, const double* value, double* cell_value);
+/* apply_cell_to_particles_avg p2mesh-kernel.cuh:256-271: cell_value/count,
+ * -1 for a cell of count 0. */
+void orc_p2mesh_cell_avg(int n, const int* cell_id, const int* count, const double* cell_value,
+                         double* out);
+/* acc_abs p2mesh-kernel.cuh:289-298 of vec3 planes (norm3, common.cuh:84-96). */
+void orc_acc_abs(int n, const double* v, double* out);
 /* (int) conversion with GPU cvt semantics: truncate, saturate, NaN -> 0. */
 int orc_sat_trunc(double v);
 

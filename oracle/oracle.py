@@ -45,6 +45,9 @@ def lib():
         L.orc_p2mesh_bin.argtypes = [_P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]
         L.orc_p2mesh_bin.restype = _I
         L.orc_p2mesh_cell_phi.argtypes = [_I, ctypes.c_longlong, _P, _P, _P, _P, _I]
+        L.orc_p2mesh_cell_weighted.argtypes = [_I, ctypes.c_longlong, _P, _P, _P, _P, _P]
+        L.orc_p2mesh_cell_avg.argtypes = [_I, _P, _P, _P, _P]
+        L.orc_acc_abs.argtypes = [_I, _P, _P]
         L.orc_sat_trunc.argtypes = [_D]
         L.orc_sat_trunc.restype = _I
         L.orc_wave_step.argtypes = [_I] + [_D] * 7 + [_P] * 4
@@ -171,6 +174,34 @@ def p2mesh_cell_phi(cell_id, count, phi, cell_phi=None, accumulate=False):
     out = np.zeros(G) if cell_phi is None else _f64(cell_phi).copy()
     lib().orc_p2mesh_cell_phi(cell_id.shape[0], G, _p(cell_id), _p(count), _p(phi), _p(out),
                               1 if accumulate else 0)
+    return out
+
+
+def p2mesh_cell_weighted(cell_id, cell_mass, mass, value):
+    """apply_particle_to_cell_weighted in index order; returns cell_value."""
+    cell_id = np.ascontiguousarray(cell_id, dtype=np.int32)
+    cell_mass, mass, value = _f64(cell_mass), _f64(mass), _f64(value)
+    out = np.zeros(cell_mass.shape[0])
+    lib().orc_p2mesh_cell_weighted(cell_id.shape[0], cell_mass.shape[0], _p(cell_id),
+                                   _p(cell_mass), _p(mass), _p(value), _p(out))
+    return out
+
+
+def p2mesh_cell_avg(cell_id, count, cell_value):
+    """apply_cell_to_particles_avg: per-body cell_value / count (-1 if empty)."""
+    cell_id = np.ascontiguousarray(cell_id, dtype=np.int32)
+    count = np.ascontiguousarray(count, dtype=np.int32)
+    cell_value = _f64(cell_value)
+    out = np.zeros(cell_id.shape[0])
+    lib().orc_p2mesh_cell_avg(cell_id.shape[0], _p(cell_id), _p(count), _p(cell_value), _p(out))
+    return out
+
+
+def acc_abs(v):
+    """acc_abs of vec3 planes (3, n): |v_i| in FP64."""
+    v = _f64(v)
+    out = np.zeros(v.shape[1])
+    lib().orc_acc_abs(v.shape[1], _p(v), _p(out))
     return out
 
 

@@ -97,3 +97,36 @@ def test_wave_solver_chain(ctx):
     assert scale > 0
     assert np.max(np.abs(a - ref_acc)) / scale <= 1e-4
     assert np.max(np.abs(phi.cpu().numpy() - ref_p)) / np.max(np.abs(ref_p)) <= 1e-5
+
+
+@pytest.mark.parametrize("nx", [2, 37, 70])
+def test_field_f64_bit_exact(ctx, nx):
+    # the FP64 form (the debug dumps' differencing of the bodies' FP64 cell
+    # potential): the same expression as the oracle in FP64; |a| within one
+    # FP64 ulp (device sqrt)
+    import torch
+    G = nx ** 3
+    dx = 4.0 / nx
+    phi = np.random.default_rng(nx + 1).normal(size=G) * 25.0
+    acc = torch.zeros((3, G), device="cuda:0", dtype=torch.float64)
+    ab = torch.zeros(G, device="cuda:0", dtype=torch.float64)
+    ctx.mesh_acc_f64(nx, dx, to_dev(phi, torch.float64), acc, ab)
+    torch.cuda.synchronize()
+    ref, ref_ab = orc.mesh_acc(nx, dx, phi)
+    np.testing.assert_array_equal(acc.cpu().numpy(), ref)
+    np.testing.assert_array_max_ulp(ab.cpu().numpy(), ref_ab, maxulp=1)
+
+
+def test_acc_abs(ctx):
+    # acc_abs (p2mesh-kernel.cuh:289-298) of FP32 planes in FP64: Pythagorean
+    # quadruples exact, the rest within one FP64 ulp of the oracle
+    import torch
+    rng = np.random.default_rng(31)
+    v = (rng.normal(size=(3, 5000)) * 10.0 ** rng.integers(-6, 6, (1, 5000))).astype(np.float32)
+    v[:, :3] = [[1, 2, 3], [2, 3, 4], [2, 6, 12]]
+    out = torch.zeros(v.shape[1], device="cuda:0", dtype=torch.float64)
+    ctx.acc_abs(to_dev(v, torch.float32), out)
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    assert list(o[:3]) == [3.0, 7.0, 13.0]
+    np.testing.assert_array_max_ulp(o, orc.acc_abs(v.astype(np.float64)), maxulp=1)

@@ -124,3 +124,66 @@ def test_rejects_bad_grid(ctx):
     g.size[0] = 0
     with pytest.raises(khrr.KhrrError, match="invalid grid"):
         _bin_gpu(ctx, g, np.zeros((3, 1)), np.ones(1))
+
+
+@pytest.mark.parametrize("nx,n", [(24, 20000), (128, 200000), (5, 1)])
+def test_cell_transfers_bit_exact(ctx, nx, n):
+    # computeCellPhiUnsorted, apply_particle_to_cell_weighted and
+    # apply_cell_to_particles_avg (p2mesh-kernel.cuh:152-167,256-287) on the
+    # outputs of a binning: index-order FP64 sums, bit for bit.
+    import torch
+    pos, _, mass, _ = galaxy_system(max(1, n // 3), n - n // 3, seed=nx) if n > 1 else (
+        np.zeros((3, 1)), None, np.array([0.5]), None)
+    n = pos.shape[1]
+    g = _grid(nx, -2.0, 2.0)
+    G = nx ** 3
+    got = _bin_gpu(ctx, g, pos, mass)
+    ref = _bin_ref(g, pos, mass)
+    _check_equal(got, ref)
+    d = {k: to_dev(v, torch.int32) for k, v in got.items() if k in (
+        "cell_id", "id_in_cell", "count", "offset")}
+    rng = np.random.default_rng(nx)
+    f64 = dict(device="cuda:0", dtype=torch.float64)
+    # cell potential of the bodies, fresh and accumulated
+    phi = (rng.normal(size=n) * 3.0).astype(np.float32)
+    cphi = torch.full((G,), 5.0, **f64)
+    ctx.p2mesh_cell_phi(d["cell_id"], d["id_in_cell"], d["count"], d["offset"],
+                        to_dev(phi, torch.float32), cphi)
+    ref_cphi = orc.p2mesh_cell_phi(ref["cell_id"], ref["count"], phi.astype(np.float64))
+    np.testing.assert_array_equal(cphi.cpu().numpy(), ref_cphi)
+    start = rng.normal(size=G)
+    acc = to_dev(start, torch.float64)
+    ctx.p2mesh_cell_phi(d["cell_id"], d["id_in_cell"], d["count"], d["offset"],
+                        to_dev(phi, torch.float32), acc, accumulate=True)
+    np.testing.assert_array_equal(acc.cpu().numpy(), orc.p2mesh_cell_phi(
+        ref["cell_id"], ref["count"], phi.astype(np.float64), cell_phi=start, accumulate=True))
+    # mass-weighted cell mean of a body value
+    value = rng.normal(size=n) * 7.0
+    cm = to_dev(got["cell_mass"], torch.float64)
+    cval = torch.full((G,), -3.0, **f64)
+    ctx.p2mesh_cell_weighted(d["cell_id"], d["id_in_cell"], d["count"], d["offset"], cm,
+                             to_dev(mass, torch.float64), to_dev(value, torch.float64), cval)
+    ref_w = orc.p2mesh_cell_weighted(ref["cell_id"], ref["cell_mass"], mass, value)
+    np.testing.assert_array_equal(cval.cpu().numpy(), ref_w)
+    # cell value shared out over the bodies
+    out = torch.full((n,), -3.0, **f64)
+    ctx.p2mesh_cell_avg(d["cell_id"], d["count"], cm, out)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), orc.p2mesh_cell_avg(
+        ref["cell_id"], ref["count"], ref["cell_mass"]))
+
+
+def test_cell_avg_empty_and_bad_cells(ctx):
+    import torch
+    G = 10
+    cid = np.array([0, 3, 3, -1, 10, 7], np.int32)
+    count = np.array([1, 0, 0, 2, 0, 0, 0, 0, 0, 0], np.int32)
+    vals = np.arange(G, dtype=np.float64) + 0.5
+    out = torch.zeros(cid.size, device="cuda:0", dtype=torch.float64)
+    ctx.p2mesh_cell_avg(to_dev(cid, torch.int32), to_dev(count, torch.int32),
+                        to_dev(vals, torch.float64), out)
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    assert o[0] == 0.5 and o[1] == 3.5 / 2 and o[2] == 3.5 / 2
+    assert np.isnan(o[3]) and np.isnan(o[4])
+    assert o[5] == -1.0

@@ -121,3 +121,103 @@ def test_wave_solver_run(tmp_path):
     for frame in ("    0", "    1", "    4"):
         assert os.path.exists(os.path.join(out, "phi", f"phi_{frame}.csv"))
     assert os.path.getsize(os.path.join(base, "bin", "LIE_0.bin")) == 72
+
+
+def _rows(path, header):
+    lines = open(path).read().splitlines()
+    assert lines[0] == header, (path, lines[0])
+    return np.array([[float(x) for x in ln.split(",")] for ln in lines[1:]])
+
+
+@pytest.mark.gpu
+def test_debug_dumps(tmp_path):
+    # convert_particles (nbody.cu:287-701) with its checks (:35-127): the
+    # files of data/nbody/debug after the initial forces, against the FP64
+    # oracle of the same chain.  Integer files exactly; the binning-only
+    # values to their printed decimals; values through the FP32 forces
+    # within 1e-4, and through the FP32 wave field within 1e-4 of their
+    # largest value (the stencil's own tolerance is 1e-5 per field).
+    import oracle as orc
+    base = _layout(str(tmp_path), [dict(n_s=1024, n_dm=2048, seed=7)], 0.02, 0.002)
+    nx, dx, iters = 16, 0.25, 50
+    r = subprocess.run([BIN, "--data", base, "--grid", str(nx), str(dx), "--debug-dumps",
+                        "--setup-iters", str(iters), "--max-steps", "1"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    dbg = os.path.join(base, "debug")
+    ps, _, _ = ics.read_start(os.path.join(base, "ini", "start_S0.txt"))
+    pd, _, _ = ics.read_start(os.path.join(base, "ini", "start_DM0.txt"))
+    pos = np.concatenate([ps, pd], axis=1)
+    n_s, n_d = ps.shape[1], pd.shape[1]
+    mass = np.concatenate([np.full(n_s, 1.0 / n_s), np.full(n_d, 6.699356 / n_d)])
+    h = np.full(n_s + n_d, 0.5 * 0.004 ** 2).astype(np.float32).astype(np.float64)
+    b = orc.p2mesh_bin([-2.0] * 3, [dx] * 3, [nx] * 3, pos, mass)
+    G = nx ** 3
+    line = np.arange(nx) + (nx // 2) * nx + (nx // 2) * nx * nx
+    count, cm, cid = b["count"], b["cell_mass"], b["cell_id"]
+    # check_particles_distribution
+    vals, freq = np.unique(count, return_counts=True)
+    d = _rows(os.path.join(dbg, "particles_in_cell.txt"),
+              "n_particles, cells_with_so_many_particles")
+    np.testing.assert_array_equal(d, np.stack([vals[::-1], freq[::-1]], axis=1))
+    part = np.cumsum(vals[::-1] * freq[::-1].astype(np.float64))
+    ps_rows = np.array([[float(x) for x in ln.split(",")] for ln in
+                        open(os.path.join(dbg, "particles_in_cell_partial_sum.txt"))])
+    np.testing.assert_array_equal(ps_rows[:, 0], np.arange(1, len(part)))
+    np.testing.assert_allclose(ps_rows[:, 1], part[1:], rtol=1e-5)
+    # binning slices and per-body mass lists
+    x = -2.0 + np.arange(nx) * dx
+    d = _rows(os.path.join(dbg, "particle_counts.txt"), "xi, counts")
+    np.testing.assert_array_equal(d, np.stack([np.arange(nx), count[line]], axis=1))
+    d = _rows(os.path.join(dbg, "mass_cell.txt"), "x, mass")
+    np.testing.assert_array_equal(d[:, 0], x)
+    np.testing.assert_allclose(d[:, 1], cm[line], atol=5.1e-7)
+    d = _rows(os.path.join(dbg, "mpart.txt"), "i, acc")
+    np.testing.assert_array_equal(d[:, 0], np.arange(pos.shape[1]))
+    np.testing.assert_allclose(d[:, 1], mass, atol=5.1e-8)
+    d = _rows(os.path.join(dbg, "mpart_from_cell.txt"), "i, acc")
+    np.testing.assert_allclose(d[:, 1], orc.p2mesh_cell_avg(cid, count, cm), atol=5.1e-8)
+    # body forces
+    acc, phi = orc.accel(pos, h, pos, mass, h)
+    pabs = orc.acc_abs(acc)
+    d = _rows(os.path.join(dbg, "nbody_part_acc.txt"), "i, acc")
+    np.testing.assert_allclose(d[:, 1], pabs, rtol=1e-4, atol=5.1e-8)
+    d = _rows(os.path.join(dbg, "nbody_cell_acc.txt"), "x, acc")
+    np.testing.assert_allclose(d[:, 1], orc.p2mesh_cell_weighted(cid, cm, mass, pabs)[line],
+                               rtol=1e-4, atol=5.1e-8)
+    cphi = orc.p2mesh_cell_phi(cid, count, phi)
+    d = _rows(os.path.join(dbg, "phi_part.txt"), "x, phi")
+    np.testing.assert_allclose(d[:, 1], cphi[line], rtol=1e-4, atol=5.1e-8)
+    _, cabs = orc.mesh_acc(nx, dx, cphi)
+    d = _rows(os.path.join(dbg, "acc_part.txt"), "x, acc")
+    assert np.max(np.abs(d[:, 1] - cabs[line])) <= 1e-4 * np.max(cabs) + 5.1e-8
+    # wave field from rest and its acceleration
+    wphi, _ = orc.wave_iterate(orc.reference_wave(nx, dx), np.zeros(G), np.zeros(G), cm, iters)
+    wacc, wabs = orc.mesh_acc(nx, dx, wphi)
+    for name, ref in (("phi_cell.txt", wphi), ("acc_cell.txt", wabs)):
+        d = _rows(os.path.join(dbg, name), "x, " + ("phi" if "phi" in name else "acc"))
+        assert np.max(np.abs(ref)) > 0
+        assert np.max(np.abs(d[:, 1] - ref[line])) <= 1e-4 * np.max(np.abs(ref)) + 5.1e-8, name
+    cpabs = orc.acc_abs(wacc[:, cid])
+    d = _rows(os.path.join(dbg, "cell_part_acc.txt"), "i, acc")
+    assert np.max(np.abs(d[:, 1] - cpabs)) <= 1e-4 * np.max(cpabs) + 5.1e-8
+    cyc = orc.p2mesh_cell_weighted(cid, cm, mass, cpabs)
+    d = _rows(os.path.join(dbg, "acc_cell_cycle.txt"), "x, acc")
+    assert np.max(np.abs(d[:, 1] - cyc[line])) <= 1e-4 * np.max(cpabs) + 5.1e-8
+    # console lines of the checks
+    out = {}
+    for ln in r.stdout.splitlines():
+        if ": " in ln:
+            k, v = ln.rsplit(": ", 1)
+            out[k] = v
+    assert "cell info mismatch!!! " not in r.stdout
+    ep = np.sum(0.5 * cm * wphi)
+    assert abs(float(out["ep"]) - ep) <= 1e-4 * np.sum(0.5 * cm * np.abs(wphi))
+    inside = cm.sum()
+    np.testing.assert_allclose(float(out["total mass in cells"]), inside, rtol=1e-5)
+    np.testing.assert_allclose(float(out["total mass in particles"]), mass.sum(), rtol=1e-5)
+    np.testing.assert_allclose(float(out["total mpart from cell"]),
+                               np.sum(orc.p2mesh_cell_avg(cid, count, cm)), rtol=1e-5)
+    np.testing.assert_allclose(float(out["total mpart"]), mass.sum(), rtol=1e-5)
+    np.testing.assert_allclose(float(out["nbody part acc sum"]), pabs.sum(), rtol=1e-4)
+    np.testing.assert_allclose(float(out["cell part acc sum"]), cpabs.sum(), rtol=1e-4)

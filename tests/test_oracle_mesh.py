@@ -65,3 +65,18 @@ def test_a_wrong_sign_would_fail():
     assert np.all(acc[0] < 0)
     with pytest.raises(AssertionError):
         np.testing.assert_allclose(acc[0], 1.0)
+
+
+def test_acc_abs_pins():
+    # acc_abs (p2mesh-kernel.cuh:289-298), norm3 of common.cuh:84-96:
+    # Pythagorean quadruples give integers exactly, signs do not matter,
+    # and the rest agrees with numpy.linalg.norm.
+    quads = np.array([[1, 2, 2, 3], [2, 3, 6, 7], [1, 4, 8, 9], [2, 6, 9, 11],
+                      [3, 4, 12, 13], [0, 0, 0, 0], [0, 0, -5, 5]], dtype=np.float64)
+    v = np.ascontiguousarray(quads[:, :3].T)
+    np.testing.assert_array_equal(orc.acc_abs(v), quads[:, 3])
+    rng = np.random.default_rng(23)
+    v = rng.normal(size=(3, 1000)) * 10.0 ** rng.integers(-8, 8, (1, 1000))
+    out = orc.acc_abs(v)
+    np.testing.assert_array_equal(orc.acc_abs(-v), out)
+    np.testing.assert_allclose(out, np.linalg.norm(v, axis=0), rtol=4e-16)

@@ -123,3 +123,65 @@ def test_cell_phi_pins():
     acc = orc.p2mesh_cell_phi(cell_id, count, phi, cell_phi=start, accumulate=True)
     np.testing.assert_allclose(acc, start + out, rtol=1e-14, atol=1e-15)
     assert np.all(acc[empty] == start[empty])
+
+
+def test_cell_weighted_pins():
+    # apply_particle_to_cell_weighted (p2mesh-kernel.cuh:273-287): every body
+    # adds value_i * m_i / M_cell, so a cell holds the mass-weighted mean of
+    # its bodies' values.  Pinned by what that fixes: one-body cells get the
+    # value exactly, four bodies of mass 1/4 sum exactly, a constant value is
+    # returned, sum_c M_c v_c = sum_i m_i v_i, empty cells stay 0, and the
+    # mean equals numpy's weighted average.  Using counts in place of masses,
+    # or another cell's mass, fails the conservation and average checks.
+    rng = np.random.default_rng(21)
+    G = 50
+    cell_id = np.concatenate([
+        np.full(4, 2), np.array([5]), np.array([9]),
+        rng.integers(10, 40, 700)]).astype(np.int32)
+    n = cell_id.shape[0]
+    mass = rng.uniform(0.1, 3.0, n)
+    mass[:4] = 0.25
+    value = rng.normal(size=n)
+    value[:4] = [1.0, 2.0, 3.0, 4.0]
+    cell_mass = np.zeros(G)
+    np.add.at(cell_mass, cell_id, mass)
+    out = orc.p2mesh_cell_weighted(cell_id, cell_mass, mass, value)
+    assert out[2] == 2.5
+    assert out[5] == value[4] and out[9] == value[5]
+    empty = np.bincount(cell_id, minlength=G) == 0
+    assert np.all(out[empty] == 0.0)
+    np.testing.assert_allclose(np.sum(cell_mass * out), np.sum(mass * value), rtol=1e-12)
+    for c in range(10, 40):
+        sel = cell_id == c
+        if sel.any():
+            np.testing.assert_allclose(out[c], np.average(value[sel], weights=mass[sel]),
+                                       rtol=1e-12, atol=1e-14)
+    const = orc.p2mesh_cell_weighted(cell_id, cell_mass, mass, np.full(n, -0.7))
+    np.testing.assert_allclose(const[~empty], -0.7, rtol=1e-14)
+    # linear in the value
+    v2 = rng.normal(size=n)
+    both = orc.p2mesh_cell_weighted(cell_id, cell_mass, mass, 2.0 * value + v2)
+    np.testing.assert_allclose(both, 2.0 * out + orc.p2mesh_cell_weighted(
+        cell_id, cell_mass, mass, v2), rtol=1e-12, atol=1e-13)
+
+
+def test_cell_avg_pins():
+    # apply_cell_to_particles_avg (p2mesh-kernel.cuh:256-271): the cell's
+    # value divided by its count at each body, -1 where the count is 0.
+    # Equal masses of 1/4 come back exactly from the binning's cell mass.
+    rng = np.random.default_rng(22)
+    G = 30
+    cell_id = rng.integers(0, 20, 400).astype(np.int32)
+    count = np.bincount(cell_id, minlength=G).astype(np.int32)
+    mass = np.full(cell_id.shape[0], 0.25)
+    cell_mass = np.zeros(G)
+    np.add.at(cell_mass, cell_id, mass)
+    out = orc.p2mesh_cell_avg(cell_id, count, cell_mass)
+    assert np.all(out == 0.25)
+    vals = rng.normal(size=G)
+    out = orc.p2mesh_cell_avg(cell_id, count, vals)
+    np.testing.assert_array_equal(out, vals[cell_id] / count[cell_id])
+    bad = count.copy()
+    bad[cell_id[0]] = 0
+    out = orc.p2mesh_cell_avg(cell_id, bad, vals)
+    assert np.all(out[cell_id == cell_id[0]] == -1.0)
