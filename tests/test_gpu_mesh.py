@@ -130,3 +130,28 @@ def test_acc_abs(ctx):
     o = out.cpu().numpy()
     assert list(o[:3]) == [3.0, 7.0, 13.0]
     np.testing.assert_array_max_ulp(o, orc.acc_abs(v.astype(np.float64)), maxulp=1)
+
+
+def test_debug_transfers_reject_bad_arguments(ctx):
+    # the entry points behind the debug dumps fail with KHRR_EINVAL, not a
+    # launch, on bad sizes and missing buffers
+    import khrr
+    import torch
+    lib = khrr.lib()
+    f64 = torch.zeros(8, device="cuda:0", dtype=torch.float64)
+    i32 = torch.zeros(8, device="cuda:0", dtype=torch.int32)
+    p = lambda t: t.data_ptr()
+    EINVAL = -1  # KHRR_EINVAL
+    assert lib.khrr_acc_abs(ctx._h, -1, None, None) == EINVAL
+    assert lib.khrr_acc_abs(ctx._h, 4, None, p(f64)) == EINVAL
+    assert lib.khrr_acc_abs(ctx._h, 0, None, None) == 0
+    assert lib.khrr_mesh_acc_f64(ctx._h, 1, 1.0, p(f64), p(f64), None) == EINVAL
+    assert lib.khrr_mesh_acc_f64(ctx._h, 2, -1.0, p(f64), p(f64), None) == EINVAL
+    assert lib.khrr_mesh_acc_f64(ctx._h, 2, 1.0, None, p(f64), None) == EINVAL
+    assert lib.khrr_p2mesh_cell_avg(ctx._h, 4, 0, p(i32), p(i32), p(f64), p(f64)) == EINVAL
+    assert lib.khrr_p2mesh_cell_avg(ctx._h, 4, 8, None, p(i32), p(f64), p(f64)) == EINVAL
+    assert lib.khrr_p2mesh_cell_weighted(ctx._h, -1, 8, p(i32), p(i32), p(i32), p(i32), p(f64),
+                                         p(f64), p(f64), p(f64)) == EINVAL
+    assert lib.khrr_p2mesh_cell_weighted(ctx._h, 4, 8, p(i32), p(i32), p(i32), p(i32), p(f64),
+                                         None, p(f64), p(f64)) == EINVAL
+    assert "null" in lib.khrr_last_error(ctx._h).decode()
