@@ -204,24 +204,31 @@ __global__ __launch_bounds__(256) void block_bounds(int n, int nb, const int* __
 }
 
 // One workgroup per 512-cell block of the reference.  Empty blocks (most
-// of a fine grid) are four coalesced stores per cell.  In the others an
-// LDS histogram and scan give counts and exact offsets; id_in_cell of the
-// particle at sorted position p is p minus its cell's offset (the sort
-// keeps index order within a cell); the block's masses are gathered into
-// LDS in chunks by all threads and each cell adds its own in index order,
-// so the FP64 sum is that of a sequential walk while the loads run in
-// parallel (a cell of many particles costs a few chunk round trips instead
-// of two dependent loads per eight particles).
+// of a fine grid) are four coalesced stores per cell, written non-temporal:
+// they are most of the kernel's traffic (20 B a cell) and would otherwise
+// evict from L2 the sorted keys, indices and masses the occupied blocks
+// read.  In an occupied block every thread first loads the cell, index and
+// mass of up to kReg of the block's bodies into registers, so those loads
+// are in flight through the histogram and the scan; an LDS histogram and
+// scan give counts and exact offsets; id_in_cell of the body at sorted
+// position p is p minus its cell's offset (the sort keeps index order
+// within a cell); the masses go to LDS at their sorted positions and each
+// cell adds its own in index order, the FP64 sum of a sequential walk.  A
+// block of more than kReg * 512 bodies repeats the last step per chunk.
+// Stand-alone timing at 256^3 with the headline galaxy: 64.3 us, against
+// 91.6 us with plain stores and one load per step.
+constexpr int kReg = 4;
+
 __global__ __launch_bounds__(kCellBlock) void bin_cells(
     int n, long long G, const int* __restrict__ skeys, const int* __restrict__ svals,
     const int* __restrict__ bounds, const double* __restrict__ mass, int* __restrict__ count,
     int* __restrict__ offset, int* __restrict__ start_ref, int* __restrict__ block_incl,
     int* __restrict__ id_in_cell, double* __restrict__ cell_mass) {
-  constexpr int kMassChunk = 2048;
+  constexpr int kChunk = kReg * kCellBlock;
   __shared__ int cnt[kCellBlock];
   __shared__ int exs[kCellBlock];
   __shared__ int wsum[kCellBlock / khrr::kWave];
-  __shared__ double msh[kMassChunk];
+  __shared__ double msh[kChunk];
   const int tid = threadIdx.x;
   const long long c0 = (long long)blockIdx.x * kCellBlock;
   const long long c = c0 + tid;
@@ -229,16 +236,32 @@ __global__ __launch_bounds__(kCellBlock) void bin_cells(
   if (tid == 0 && block_incl) block_incl[blockIdx.x] = hi;
   if (lo == hi) {
     if (c < G) {
-      count[c] = 0;
-      if (offset) offset[c] = lo;
-      if (start_ref) start_ref[c] = lo;
-      if (cell_mass) cell_mass[c] = 0.0;
+      __builtin_nontemporal_store(0, &count[c]);
+      if (offset) __builtin_nontemporal_store(lo, &offset[c]);
+      if (start_ref) __builtin_nontemporal_store(lo, &start_ref[c]);
+      if (cell_mass) __builtin_nontemporal_store(0.0, &cell_mass[c]);
     }
     return;
   }
+  // the first chunk's cells (relative to the block), indices and masses
+  int key[kReg], val[kReg];
+  double mv[kReg];
+#pragma unroll
+  for (int u = 0; u < kReg; ++u) {
+    const int p = lo + u * kCellBlock + tid;
+    key[u] = p < hi ? skeys[p] - (int)c0 : -1;
+    val[u] = p < hi ? svals[p] : 0;
+  }
+  if (cell_mass) {
+#pragma unroll
+    for (int u = 0; u < kReg; ++u) mv[u] = key[u] >= 0 ? mass[val[u]] : 0.0;
+  }
   cnt[tid] = 0;
   __syncthreads();
-  for (int p = lo + tid; p < hi; p += kCellBlock) atomicAdd(&cnt[skeys[p] - (int)c0], 1);
+#pragma unroll
+  for (int u = 0; u < kReg; ++u)
+    if (key[u] >= 0) atomicAdd(&cnt[key[u]], 1);
+  for (int p = lo + kChunk + tid; p < hi; p += kCellBlock) atomicAdd(&cnt[skeys[p] - (int)c0], 1);
   __syncthreads();
   // Exclusive scan of cnt: wave-inclusive scans, then wave totals.
   const int lane = tid & (khrr::kWave - 1);
@@ -262,19 +285,35 @@ __global__ __launch_bounds__(kCellBlock) void bin_cells(
     if (start_ref) start_ref[c] = lo;
   }
   __syncthreads();
-  if (id_in_cell)
-    for (int p = lo + tid; p < hi; p += kCellBlock) id_in_cell[svals[p]] = p - exs[skeys[p] - (int)c0];
-  if (!cell_mass) return;  // the same for the whole workgroup
   double m = 0.0;
-  for (int p0 = lo; p0 < hi; p0 += kMassChunk) {
-    const int p1 = min(hi, p0 + kMassChunk);
-    for (int p = p0 + tid; p < p1; p += kCellBlock) msh[p - p0] = mass[svals[p]];
+  for (int p0 = lo; p0 < hi; p0 += kChunk) {
+    if (p0 != lo) {  // chunks after the first (blocks of more than kChunk bodies)
+#pragma unroll
+      for (int u = 0; u < kReg; ++u) {
+        const int p = p0 + u * kCellBlock + tid;
+        key[u] = p < hi ? skeys[p] - (int)c0 : -1;
+        val[u] = p < hi ? svals[p] : 0;
+      }
+      if (cell_mass) {
+#pragma unroll
+        for (int u = 0; u < kReg; ++u) mv[u] = key[u] >= 0 ? mass[val[u]] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kReg; ++u) {
+      if (key[u] < 0) continue;
+      const int p = p0 + u * kCellBlock + tid;
+      if (id_in_cell) id_in_cell[val[u]] = p - exs[key[u]];
+      if (cell_mass) msh[u * kCellBlock + tid] = mv[u];
+    }
+    if (!cell_mass) continue;  // the same for the whole workgroup
     __syncthreads();
+    const int p1 = min(hi, p0 + kChunk);
     const int q1 = min(excl + mine, p1);
     for (int q = max(excl, p0); q < q1; ++q) m += msh[q - p0];
     __syncthreads();  // msh is refilled by the next chunk
   }
-  if (c < G) cell_mass[c] = m;
+  if (c < G && cell_mass) cell_mass[c] = m;
 }
 
 // perm[offset[cell_id[i]] + id_in_cell[i]] = i: the sorted segments back
