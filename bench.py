@@ -4,9 +4,10 @@
 Workload (one GPU): one disk + halo galaxy, 262144 stars (mass 1.0) and
 524288 halo bodies (mass 6.699356), eps 0.004, dt 0.002, FP32 direct-sum
 gravity with FP64 predictor-corrector state.  A step is the reference's
-predictor -> ACCEL -> corrector (nbody.cu:1525-1582) over all 786432 bodies;
-`value` is whole-step pair-interactions per second (N^2 per step, i = j
-counted as the reference does), inputs resident in HBM.
+predictor -> ACCEL -> corrector (nbody.cu:1525-1582) over all 786432 bodies,
+50 timed steps by default after 2 warm-up steps; `value` is whole-step
+pair-interactions per second (N^2 per step, i = j counted as the reference
+does), inputs resident in HBM.
 
 Also reported on the same line (one GPU):
   roofline      gravity kernel: 20 FLOP per pair vs the 157.3 TFLOP/s FP32
@@ -54,7 +55,7 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=50)  # the headline config's 50 steps
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n-stars", type=int, default=262144)
     ap.add_argument("--n-dm", type=int, default=524288)
