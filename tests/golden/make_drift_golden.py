@@ -1,4 +1,4 @@
-"""Writes tests/golden/c0_drift_golden.npz: the FP64 oracle's run of the
+"""Writes tests/golden/galaxy24k_drift_golden.npz: the FP64 oracle's run of the
 small plain-method workload, for the drift comparison of the GPU step loop
 (run once, in the build container; the output is committed, the GPU host
 never runs this).
@@ -43,7 +43,7 @@ def main():
     t0 = time.perf_counter()
     _, _, d = orc.run(pos, vel, m32, h32, DT, STEPS, diag_every=EVERY)
     print(f"oracle: {STEPS} steps of n={pos.shape[1]} in {time.perf_counter() - t0:.1f} s")
-    np.savez_compressed(os.path.join(HERE, "c0_drift_golden.npz"), diag=d, dt=DT, steps=STEPS,
+    np.savez_compressed(os.path.join(HERE, "galaxy24k_drift_golden.npz"), diag=d, dt=DT, steps=STEPS,
                         every=EVERY, n_s=N_S, n_dm=N_DM, seed=SEED,
                         ic_checksum=np.array([pos.sum(), vel.sum(), (pos * pos).sum(),
                                               (vel * vel).sum(), m32.sum(), h32.sum()]))

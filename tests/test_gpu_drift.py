@@ -1,6 +1,6 @@
 """Conservation over the small plain-method workload: energy, momentum and
 angular-momentum drift of the GPU step loop against the FP64 oracle's run of
-the same bodies and steps (tests/golden/c0_drift_golden.npz, written by
+the same bodies and steps (tests/golden/galaxy24k_drift_golden.npz, written by
 make_drift_golden.py).
 
 Workload: one galaxy without external potential, 8192 disk stars + 16384
@@ -30,7 +30,7 @@ from helpers import galaxy_system, to_dev
 
 pytestmark = pytest.mark.gpu
 
-FIXTURE = os.path.join(GOLDEN, "c0_drift_golden.npz")
+FIXTURE = os.path.join(GOLDEN, "galaxy24k_drift_golden.npz")
 NAMES = ("Ek", "Ep", "Px", "Py", "Pz", "Lx", "Ly", "Lz")
 
 
@@ -79,7 +79,7 @@ def _drifts(rows):
 
 
 @pytest.mark.parametrize("compensated", [False, True], ids=["plain", "compensated"])
-def test_c0_drift_within_oracle(ctx, compensated):
+def test_galaxy24k_drift_within_oracle(ctx, compensated):
     g = np.load(FIXTURE)
     pos, vel, mass, h = galaxy_system(int(g["n_s"]), int(g["n_dm"]), seed=int(g["seed"]))
     m32 = mass.astype(np.float32).astype(np.float64)

@@ -42,8 +42,8 @@ def _gpu_accel(ctx, p32, m32, h32, rows=None, **kw):
     return acc.cpu().numpy().astype(np.float64), phi.cpu().numpy().astype(np.float64)
 
 
-def test_galaxy_c0_shape_all_rows(ctx):
-    # configs[0] shape: 8192 stars + 16384 halo bodies, every row checked
+def test_galaxy24k_all_rows(ctx):
+    # the 8192 + 16384-body galaxy, every row checked
     pos, _, mass, h = galaxy_system(8192, 16384, seed=12345)
     p32, m32, h32 = _f32_inputs(pos, mass, h)
     acc, phi = _gpu_accel(ctx, p32, m32, h32)

@@ -57,11 +57,11 @@ def test_kepler_orbit_against_mpmath_golden():
     assert np.max(np.abs(d[:, 2:5])) < 1e-15
 
 
-def test_c0_drift_fixture_is_the_oracle_run():
+def test_galaxy24k_drift_fixture_is_the_oracle_run():
     # the committed 500-step drift fixture (make_drift_golden.py) holds the
     # oracle's own run of these bodies: same initial conditions, and its
     # first two rows (t = 0 and after 10 steps) equal a fresh run bit for bit
-    g = np.load(os.path.join(GOLDEN, "c0_drift_golden.npz"))
+    g = np.load(os.path.join(GOLDEN, "galaxy24k_drift_golden.npz"))
     from helpers import galaxy_system
     pos, vel, mass, h = galaxy_system(int(g["n_s"]), int(g["n_dm"]), seed=int(g["seed"]))
     m32 = mass.astype(np.float32).astype(np.float64)
