@@ -15,6 +15,8 @@ Also reported on the same line (one GPU):
                 stream; traffic from rocprof counters when supplied
   binning       p2mesh of the final state on 256^3 over [-2,2]^3
   stencil       200 damped-wave iterations on that grid from phi = 0
+  mesh          the mesh-side entry points on that grid: the wave solver's
+                mesh force, the bodies' cell potential, the debug transfers
   batched       512 Plummer spheres (N log-uniform 1024..131072) per step,
                 one persistent launch vs one plain launch per system
   serving       a Poisson stream of mixed requests through the server
@@ -71,7 +73,7 @@ def parse():
                     help="range of the requests' integration times tmax (uniform), lo,hi")
     ap.add_argument("--serve-saturated", type=float, default=50.0,
                     help="arrival rate of the saturated serving run (0: skip it)")
-    ap.add_argument("--legs", default="binning,stencil,batched,serving,precision",
+    ap.add_argument("--legs", default="binning,stencil,mesh,batched,serving,precision",
                     help="extras to run after the headline (comma list; also batched-run, "
                          "the 512 systems advanced --batched-run-steps steps)")
     ap.add_argument("--precision-steps", type=int, default=3,
@@ -210,7 +212,7 @@ def bench_binning(ctx, sim, reps=5):
     # id_in_cell (4) written; per cell count, offset, start_id_ref (4 each)
     # and cell_mass (8) written
     alg = 40 * n + 20 * G
-    return bufs["cell_mass"], {
+    return bufs, {
         "grid": "256^3 over [-2,2]^3", "n": n, "ms": avg_ms, "alg_bytes": alg,
         "alg_GBps": alg / (avg_ms * 1e-3) / 1e9,
         "frac_hbm_peak": alg / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
@@ -241,10 +243,72 @@ def bench_stencil(ctx, cell_mass, check, iters=200):
     alg = 16 * G
     check.update(nx=nx, iters=iters, cell_mass=cell_mass.cpu().numpy(),
                  phi=cur.cpu().numpy().astype(np.float64))
-    return {"grid": "256^3", "iterations": iters, "ms_per_iter": avg_ms, "alg_bytes_per_iter": alg,
-            "alg_GBps": alg / (avg_ms * 1e-3) / 1e9,
-            "frac_hbm_peak": alg / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
-            "max_abs_phi": float(cur.abs().max().item())}
+    return cur, {"grid": "256^3", "iterations": iters, "ms_per_iter": avg_ms,
+                 "alg_bytes_per_iter": alg, "alg_GBps": alg / (avg_ms * 1e-3) / 1e9,
+                 "frac_hbm_peak": alg / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                 "max_abs_phi": float(cur.abs().max().item())}
+
+
+def bench_mesh(ctx, sim, bins, field, reps=10):
+    """The mesh-side entry points on the binning and the stencil field of
+    the headline grid (256^3, 786432 bodies): the wave solver's mesh force
+    (acc_field and the transfer to the bodies), the bodies' cell potential,
+    and the transfers of the debug dumps.  Algorithmic bytes are each entry
+    point's own inputs and outputs (scratch excluded); body values that the
+    legs before do not produce (the bodies' potential, a per-body value)
+    are seeded random, which the timing does not depend on."""
+    import numpy as np
+    import torch
+    import khrr
+    nx = 256
+    G = nx ** 3
+    dx = 4.0 / nx
+    n = sim.n
+    rng = np.random.default_rng(7)
+    f32 = dict(device="cuda:0", dtype=torch.float32)
+    f64 = dict(device="cuda:0", dtype=torch.float64)
+    phi_b = torch.as_tensor(rng.normal(size=n).astype(np.float32)).to("cuda:0")
+    val_b = torch.as_tensor(rng.normal(size=n)).to("cuda:0")
+    cacc = torch.zeros((3, G), **f32)
+    cabs = torch.zeros(G, **f32)
+    pacc = torch.zeros((3, n), **f32)
+    pphi = torch.zeros(n, **f32)
+    cphi64 = torch.zeros(G, **f64)
+    cacc64 = torch.zeros((3, G), **f64)
+    cabs64 = torch.zeros(G, **f64)
+    cval = torch.zeros(G, **f64)
+    bout = torch.zeros(n, **f64)
+    b = bins
+    legs = [
+        ("mesh_acc", khrr.K_WAVE, 20 * G,
+         lambda: ctx.mesh_acc(nx, dx, field, cacc, cabs)),
+        ("mesh_to_particles", khrr.K_BIN, 36 * n,
+         lambda: ctx.mesh_to_particles(b["cell_id"], cacc, field, pacc, pphi)),
+        ("p2mesh_cell_phi", khrr.K_BIN, 12 * n + 16 * G,
+         lambda: ctx.p2mesh_cell_phi(b["cell_id"], b["id_in_cell"], b["count"], b["offset"],
+                                     phi_b, cphi64)),
+        ("p2mesh_cell_weighted", khrr.K_BIN, 24 * n + 24 * G,
+         lambda: ctx.p2mesh_cell_weighted(b["cell_id"], b["id_in_cell"], b["count"],
+                                          b["offset"], b["cell_mass"], sim.mass64, val_b, cval)),
+        ("p2mesh_cell_avg", khrr.K_BIN, 24 * n,
+         lambda: ctx.p2mesh_cell_avg(b["cell_id"], b["count"], b["cell_mass"], bout)),
+        ("acc_abs", khrr.K_BIN, 20 * n, lambda: ctx.acc_abs(pacc, bout)),
+        ("mesh_acc_f64", khrr.K_WAVE, 40 * G,
+         lambda: ctx.mesh_acc_f64(nx, dx, cphi64, cacc64, cabs64)),
+    ]
+    out = {}
+    for name, cls, alg, call in legs:
+        call()
+        torch.cuda.synchronize()
+        ctx.reset_timing()
+        for _ in range(reps):
+            call()
+        launches, ms = ctx.kernel_time(cls)
+        avg_ms = ms / max(launches, 1)
+        gbs = alg / (avg_ms * 1e-3) / 1e9
+        out[name] = {"ms": avg_ms, "alg_bytes": alg, "alg_GBps": gbs,
+                     "frac_hbm_peak": gbs / PEAK_HBM_GBS}
+    return {"grid": "256^3 over [-2,2]^3", "n": n, "entry_points": out}
 
 
 def bench_batched(ctx, steps):
@@ -759,15 +823,19 @@ def main():
     legs = set() if args.quick else set(args.legs.split(","))
     if legs:
         ctx.set_timing(True)
-    if legs & {"binning", "stencil"}:
+    if legs & {"binning", "stencil", "mesh"}:
         log("binning")
-        cell_mass, binning = bench_binning(ctx, sim)
+        bins, binning = bench_binning(ctx, sim)
         result["binning"] = binning
-        if "stencil" in legs:
+        if legs & {"stencil", "mesh"}:
             log("stencil")
             checks["stencil"] = {}
-            result["stencil"] = bench_stencil(ctx, cell_mass, checks["stencil"])
-        del cell_mass
+            field, result["stencil"] = bench_stencil(ctx, bins["cell_mass"], checks["stencil"])
+            if "mesh" in legs:
+                log("mesh-side entry points")
+                result["mesh"] = bench_mesh(ctx, sim, bins, field)
+            del field
+        del bins
         torch.cuda.empty_cache()
     if "batched" in legs:
         log("batched")

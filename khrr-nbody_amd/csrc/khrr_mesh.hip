@@ -68,10 +68,12 @@ __global__ __launch_bounds__(kMX* kMY) void mesh_acc_kernel(int nx, double coef,
     dx *= coef;
     dy *= coef;
     dz *= coef;
-    acc[at] = (T)dx;
-    acc[G + at] = (T)dy;
-    acc[2 * G + at] = (T)dz;
-    if (acc_abs) acc_abs[at] = (T)sqrt(dx * dx + dy * dy + dz * dz);
+    // Non-temporal: the outputs are not read again here, and kept out of
+    // L2 they leave it to the field planes the neighbouring columns re-read.
+    __builtin_nontemporal_store((T)dx, &acc[at]);
+    __builtin_nontemporal_store((T)dy, &acc[G + at]);
+    __builtin_nontemporal_store((T)dz, &acc[2 * G + at]);
+    if (acc_abs) __builtin_nontemporal_store((T)sqrt(dx * dx + dy * dy + dz * dz), &acc_abs[at]);
     cm = cc;
     cc = cp;
   }

@@ -350,7 +350,9 @@ __global__ __launch_bounds__(256) void cell_phi_kernel(int n, long long G,
     const int i = perm[o + q];
     if (i >= 0 && i < n) v += (double)phi[i] / k;
   }
-  cell_phi[c] = v;
+  // non-temporal, as the per-cell outputs of the binning: most cells are
+  // empty, and this stream would evict the segments the others read
+  __builtin_nontemporal_store(v, &cell_phi[c]);
 }
 
 // apply_particle_to_cell_weighted (p2mesh-kernel.cuh:273-287) in index
@@ -376,7 +378,7 @@ __global__ __launch_bounds__(256) void cell_weighted_kernel(int n, long long G,
       if (i >= 0 && i < n) v += value[i] * (mass[i] / cm);
     }
   }
-  cell_value[c] = v;
+  __builtin_nontemporal_store(v, &cell_value[c]);
 }
 
 // apply_cell_to_particles_avg (p2mesh-kernel.cuh:256-271): the cell's value
