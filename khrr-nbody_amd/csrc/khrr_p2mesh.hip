@@ -332,6 +332,15 @@ __global__ __launch_bounds__(256) void segments_from_bins(int n, long long G,
   if (p >= 0 && p < n) perm[p] = i;
 }
 
+// A cell's walk over its sorted segment [o, o + k) stops where the segment
+// leaves [0, n) (only an inconsistent binning does that); this is the
+// number of entries it visits.
+__device__ __forceinline__ int segment_len(long long o, int k, int n) {
+  if (o < 0 || o >= n) return 0;
+  return (int)min((long long)k, (long long)n - o);
+}
+constexpr int kWalk = 4;
+
 // computeCellPhiUnsorted (p2mesh-kernel.cuh:152-167) in index order.
 __global__ __launch_bounds__(256) void cell_phi_kernel(int n, long long G,
                                                        const int* __restrict__ count,
@@ -344,11 +353,25 @@ __global__ __launch_bounds__(256) void cell_phi_kernel(int n, long long G,
   if (c >= G) return;
   const int k = count[c];
   double v = accumulate ? cell_phi[c] : 0.0;
-  const long long o = k > 0 ? offset[c] : 0;
-  for (int q = 0; q < k; ++q) {
-    if (o + q < 0 || o + q >= n) break;
-    const int i = perm[o + q];
-    if (i >= 0 && i < n) v += (double)phi[i] / k;
+  if (k > 0) {
+    const long long o = offset[c];
+    const int kk = segment_len(o, k, n);
+    int q = 0;
+    for (; q + kWalk <= kk; q += kWalk) {  // kWalk loads in flight, adds in order
+      int i[kWalk];
+      float x[kWalk];
+#pragma unroll
+      for (int u = 0; u < kWalk; ++u) i[u] = perm[o + q + u];
+#pragma unroll
+      for (int u = 0; u < kWalk; ++u) x[u] = (i[u] >= 0 && i[u] < n) ? phi[i[u]] : 0.f;
+#pragma unroll
+      for (int u = 0; u < kWalk; ++u)
+        if (i[u] >= 0 && i[u] < n) v += (double)x[u] / k;
+    }
+    for (; q < kk; ++q) {
+      const int i = perm[o + q];
+      if (i >= 0 && i < n) v += (double)phi[i] / k;
+    }
   }
   // non-temporal, as the per-cell outputs of the binning: most cells are
   // empty, and this stream would evict the segments the others read
@@ -372,8 +395,24 @@ __global__ __launch_bounds__(256) void cell_weighted_kernel(int n, long long G,
   if (k > 0) {
     const double cm = cell_mass[c];
     const long long o = offset[c];
-    for (int q = 0; q < k; ++q) {
-      if (o + q < 0 || o + q >= n) break;
+    const int kk = segment_len(o, k, n);
+    int q = 0;
+    for (; q + kWalk <= kk; q += kWalk) {  // kWalk loads in flight, adds in order
+      int i[kWalk];
+      double x[kWalk], m[kWalk];
+#pragma unroll
+      for (int u = 0; u < kWalk; ++u) i[u] = perm[o + q + u];
+#pragma unroll
+      for (int u = 0; u < kWalk; ++u) {
+        const bool ok = i[u] >= 0 && i[u] < n;
+        x[u] = ok ? value[i[u]] : 0.0;
+        m[u] = ok ? mass[i[u]] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kWalk; ++u)
+        if (i[u] >= 0 && i[u] < n) v += x[u] * (m[u] / cm);
+    }
+    for (; q < kk; ++q) {
       const int i = perm[o + q];
       if (i >= 0 && i < n) v += value[i] * (mass[i] / cm);
     }
