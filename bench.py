@@ -137,6 +137,11 @@ def energy(d):
     return 0.5 * (d["Ek"] + d["Ep"])
 
 
+def hbm_figure(ms, alg_bytes):
+    gbs = alg_bytes / (ms * 1e-3) / 1e9
+    return {"ms": ms, "alg_bytes": alg_bytes, "alg_GBps": gbs, "frac_hbm_peak": gbs / PEAK_HBM_GBS}
+
+
 def bench_precision(ctx, steps, check, samples=8192):
     """Two-galaxy encounter (2 x (524288 + 1048576) = 3145728 bodies): plain
     FP32 sums vs compensated (TwoSum) sums: time per force evaluation and
@@ -727,6 +732,11 @@ def main():
     t1 = time.perf_counter()
     grav_launches, grav_ms = ctx.kernel_time(khrr.K_GRAVITY)
     int_launches, int_ms = ctx.kernel_time(khrr.K_INTEGRATE)
+    # the conservation-law reductions, timed on their own
+    ctx.reset_timing()
+    for _ in range(10):
+        sim.diagnostics()
+    diag_calls, diag_ms = ctx.kernel_time(khrr.K_DIAG)
     ctx.set_timing(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     pairs = float(n) * n
@@ -806,6 +816,14 @@ def main():
                               f"{executed / pairs:.4f} N^2 pair evaluations"),
                      "one_sided": one_sided},
         "integrator_ms_per_launch": int_ms / max(int_launches, 1),
+        # algorithmic bytes per body: predictor 120 (reads pos, vel, acc;
+        # writes pos_t, vel_t and the FP32 position copy), corrector 144
+        # (reads pos_t, vel_t, vel, acc; writes pos, vel and the force
+        # copy), 132 on average over the two launches of a step
+        "integrator": hbm_figure(int_ms / max(int_launches, 1), 132 * n),
+        # FP64 reductions of khrr_diag (its two kernels): read pos, vel,
+        # mass, phi, 60 B per body
+        "diagnostics": hbm_figure(diag_ms / max(diag_calls, 1), 60 * n),
         "drift": drift,
     }
     # what the CPU leg checks, collected by the GPU legs (host copies)

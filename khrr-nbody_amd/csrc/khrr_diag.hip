@@ -52,20 +52,30 @@ __device__ __forceinline__ void accumulate_one(int i, size_t N, const double* __
   v[16] = fmax(v[16], fabs(z));
 }
 
+// Fixed-order reduction of the block's kVals values: a shuffle tree inside
+// each wave, then the waves' results combined in wave order by one thread
+// per value, so the result does not depend on timing.
 __device__ void block_reduce(double (&v)[kVals], double* __restrict__ out) {
-  __shared__ double sh[kThreads];
+  constexpr int kWaves = kThreads / khrr::kWave;
+  __shared__ double sh[kWaves][kVals];
+  const int lane = threadIdx.x & (khrr::kWave - 1);
+  const int w = threadIdx.x / khrr::kWave;
+#pragma unroll
   for (int k = 0; k < kVals; ++k) {
-    sh[threadIdx.x] = v[k];
-    __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) {
-        const double a = sh[threadIdx.x], b = sh[threadIdx.x + s];
-        sh[threadIdx.x] = k < kSums ? a + b : fmax(a, b);
-      }
-      __syncthreads();
+    double x = v[k];
+#pragma unroll
+    for (int off = khrr::kWave / 2; off > 0; off >>= 1) {
+      const double y = __shfl_down(x, off, khrr::kWave);
+      x = k < kSums ? x + y : fmax(x, y);
     }
-    if (threadIdx.x == 0) out[k] = sh[0];
-    __syncthreads();
+    if (lane == 0) sh[w][k] = x;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < kVals) {
+    const int k = threadIdx.x;
+    double x = sh[0][k];
+    for (int q = 1; q < kWaves; ++q) x = k < kSums ? x + sh[q][k] : fmax(x, sh[q][k]);
+    out[k] = x;
   }
 }
 
