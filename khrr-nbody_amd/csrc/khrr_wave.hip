@@ -12,7 +12,7 @@
 // tables are built on the device in FP64 and rounded to FP32.
 //
 // Layout: x fastest (AT(x,y,z) = x + y*NX + z*NX^2, common.cuh:77).  A
-// 64x4 thread tile marches along z keeping phi(z-1), phi(z), phi(z+1) in
+// 64x16 thread tile marches along z keeping phi(z-1), phi(z), phi(z+1) in
 // registers, so each cell reads phi, phi_old and the source once from HBM
 // and writes phi' once: 16 B/cell/iteration.  The x/y neighbours are
 // re-reads of lines the tile already pulled (L1/L2 hits).
@@ -27,7 +27,9 @@
 namespace {
 
 constexpr double kPi = 3.14159265358979;  // common.cuh:7
-constexpr int kTX = 64, kTY = 4, kZC = 32;
+// One-step kernel: a 64 x 16 column tile marching 16 planes (47.0 us per
+// iteration at 256^3, against 55.0 us for 64 x 4 columns marching 32).
+constexpr int kTX = 64, kTY = 16, kZC = 16;
 // Two-step kernel: output tile kFX x kFY x kFZ; first-step footprint with a
 // one-cell ring, input plane with a two-cell ring.  At 256^3 a 64 x 32 tile
 // marching 16 planes with 768 threads (66 KB of LDS, two workgroups per
