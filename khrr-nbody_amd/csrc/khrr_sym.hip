@@ -245,6 +245,24 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_items_kernel(
 }
 
 // acc[i] (+)= sum_X part[X][0..2][i];  phi[i] (-)= sum_X part[X][3][i]
+// Folds slots p[0], p[stride], ... p[(count-1)*stride] in slot order, with
+// eight loads in flight before their adds (the order of the adds, and so
+// the result, is that of a one-at-a-time walk).
+template <bool COMP>
+__device__ __forceinline__ void fold_slots(const float* __restrict__ p, size_t stride, int count,
+                                           float& hi, float& lo) {
+  constexpr int kU = 8;
+  int X = 0;
+  for (; X + kU <= count; X += kU) {
+    float t[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) t[u] = p[(size_t)(X + u) * stride];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) fold(hi, lo, t[u], COMP);
+  }
+  for (; X < count; ++X) fold(hi, lo, p[(size_t)X * stride], COMP);
+}
+
 template <bool COMP>
 __global__ __launch_bounds__(256) void sym_reduce(int n, int n_sb, int do_acc, int do_phi,
                                                   const float* __restrict__ part,
@@ -256,7 +274,7 @@ __global__ __launch_bounds__(256) void sym_reduce(int n, int n_sb, int do_acc, i
   for (int c = 0; c < 4; ++c) {
     if ((c < 3 && !do_acc) || (c == 3 && !do_phi)) continue;
     float hi = 0.f, lo = 0.f;
-    for (int X = 0; X < n_sb; ++X) fold(hi, lo, part[((size_t)X * 4 + c) * N + i], COMP);
+    fold_slots<COMP>(part + (size_t)c * N + i, 4 * N, n_sb, hi, lo);
     const float v = COMP ? hi + lo : hi;
     if (c < 3) {
       float* a = acc + c * N + i;
@@ -282,7 +300,7 @@ __global__ __launch_bounds__(256) void sym_reduce_sys(const khrr::SymSys* __rest
   for (int c = 0; c < 4; ++c) {
     if ((c < 3 && !do_acc) || (c == 3 && !do_phi)) continue;
     float hi = 0.f, lo = 0.f;
-    for (int X = 0; X < S.n_sb; ++X) fold(hi, lo, part[((size_t)X * 4 + c) * N + i], COMP);
+    fold_slots<COMP>(part + (size_t)c * N + i, 4 * N, S.n_sb, hi, lo);
     const float v = COMP ? hi + lo : hi;
     const size_t g = (size_t)S.base + i;
     if (c < 3) {
