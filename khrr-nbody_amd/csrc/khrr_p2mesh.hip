@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void block_bounds(int n, int nb, const int* __
 constexpr int kReg = 4;
 
 __global__ __launch_bounds__(kCellBlock) void bin_cells(
-    int n, long long G, const int* __restrict__ skeys, const int* __restrict__ svals,
+    long long G, const int* __restrict__ skeys, const int* __restrict__ svals,
     const int* __restrict__ bounds, const double* __restrict__ mass, int* __restrict__ count,
     int* __restrict__ offset, int* __restrict__ start_ref, int* __restrict__ block_incl,
     int* __restrict__ id_in_cell, double* __restrict__ cell_mass) {
@@ -588,8 +588,8 @@ extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, cons
   }
   hipLaunchKernelGGL(block_bounds, dim3(ceil_div(cblocks + 1, 256)), dim3(256), 0, ctx->stream, n,
                      cblocks, skeys, bounds);
-  hipLaunchKernelGGL(bin_cells, dim3(cblocks), dim3(kCellBlock), 0, ctx->stream, n, G, skeys,
-                     svals, bounds, mass, count, offset, start_id_ref, block_incl, id_in_cell,
+  hipLaunchKernelGGL(bin_cells, dim3(cblocks), dim3(kCellBlock), 0, ctx->stream, G, skeys, svals,
+                     bounds, mass, count, offset, start_id_ref, block_incl, id_in_cell,
                      cell_mass);
   return check_launch(ctx, "p2mesh kernels");
 }
