@@ -14,7 +14,10 @@ CPU:
   iterations against the FP64 oracle (max-norm relative error <= 1e-5);
 * the batched form on the full 512-system composition (13.9M bodies, N
   log-uniform in [1024, 131072]): systems across the size range equal their
-  plain single-system launches bit for bit."""
+  plain single-system launches bit for bit;
+* the precision form on the 3145728-body encounter: plain and compensated
+  sums within the force bar on 256 sampled rows, the compensated ones at
+  least as accurate."""
 import numpy as np
 import pytest
 
@@ -130,3 +133,29 @@ def test_batched_full_composition(ctx):
                   acc=o, tune=_plain_tune(n))
         torch.cuda.synchronize()
         np.testing.assert_array_equal(a[:, a0:b0], o.cpu().numpy(), err_msg=f"system {s}, n={n}")
+
+
+def test_precision_encounter(ctx):
+    # the precision form on its full workload (two galaxies, 3145728 bodies):
+    # plain and compensated sums both within the force bar on sampled rows,
+    # the compensated ones at least as accurate (median and p99)
+    import ics
+    import torch
+    pos, _, mass, h = ics.encounter()
+    n = pos.shape[1]
+    p32, m32, h32 = (a.astype(np.float32) for a in (pos, mass, h))
+    p, m, hh = to_dev(p32, torch.float32), to_dev(m32, torch.float32), to_dev(h32, torch.float32)
+    rows = np.sort(np.random.default_rng(3145).choice(n, 256, replace=False)).astype(np.int32)
+    ref, _ = orc.accel_rows(rows, p32.astype(np.float64), h32.astype(np.float64),
+                            m32.astype(np.float64))
+    stats = {}
+    for comp in (False, True):
+        acc = torch.zeros((3, n), device="cuda:0")
+        ctx.accel(p, hh, p, m, hh, acc=acc, compensated=comp)
+        torch.cuda.synchronize()
+        a = acc.cpu().numpy().astype(np.float64)
+        stats[comp] = err_stats(rel_err(a[:, rows], ref))
+        del acc
+        assert stats[comp]["median"] <= 1e-6 and stats[comp]["p99"] <= 1e-5, (comp, stats[comp])
+    assert stats[True]["median"] <= stats[False]["median"], stats
+    assert stats[True]["p99"] <= stats[False]["p99"], stats
