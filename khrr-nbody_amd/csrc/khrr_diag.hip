@@ -128,7 +128,8 @@ __global__ __launch_bounds__(kThreads) void diag_partial(int n, int per,
     Body b[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u)
-      if (o + u * kThreads < len) b[u] = load_body(i0 + o + u * kThreads, (size_t)n, pos, vel, mass, phi);
+      if (o + u * kThreads < len)
+        b[u] = load_body(i0 + o + u * kThreads, (size_t)n, pos, vel, mass, phi);
 #pragma unroll
     for (int u = 0; u < kU; ++u)
       if (o + u * kThreads < len) accumulate_one(b[u], has_phi, v);
