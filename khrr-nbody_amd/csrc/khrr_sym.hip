@@ -141,8 +141,10 @@ __device__ __forceinline__ void sym_block(
       khrr::tile_step<IPL, DO_ACC, DO_PHI, COMP>(jt, min(j1, jt + 256), sb_flags[A], xs, ys, zs,
                                                  m, h, xi, yi, zi, hi, acc);
   } else {
-    const bool uh = (flags & kUniformH) != 0;
-    const bool um = (flags & kUniformM) != 0;
+    // Only the specialised loop (softening and mass both uniform over A and
+    // B) leaves the masses out of the pair terms; the generic loop applies
+    // m_j and m_i per pair, so its sums take no scale afterwards.
+    const bool uni = (flags & kUniformH) != 0 && (flags & kUniformM) != 0;
     const int j0 = B * SB, j1 = min(n, j0 + SB);
     const float hsum = h[A * SB] + h[j0];
     const float mA = m[A * SB], mB = m[j0];
@@ -158,13 +160,13 @@ __device__ __forceinline__ void sym_block(
 #pragma unroll
       for (int k = 0; k < IPL; ++k) tx[k] = ty[k] = tz[k] = tp[k] = 0.f;
       float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
-      if (uh && um)
+      if (uni)
         chunk64<IPL, DO_ACC, DO_PHI, true, true>(xj, yj, zj, mj, hj, xi, yi, zi, hh, mi, tx, ty,
                                                  tz, tp, bx, by, bz, bp);
       else
         chunk64<IPL, DO_ACC, DO_PHI, false, false>(xj, yj, zj, mj, hj, xi, yi, zi, hi, mi, tx, ty,
                                                    tz, tp, bx, by, bz, bp);
-      const float si = um ? mB : 1.f;
+      const float si = uni ? mB : 1.f;
 #pragma unroll
       for (int k = 0; k < IPL; ++k) {
         if (DO_ACC) {
@@ -187,7 +189,7 @@ __device__ __forceinline__ void sym_block(
         if (want && c0 + l < j1) {
           float s = 0.f;
           for (int w = 0; w < W; ++w) s += red[w][c][l];
-          const float sj = um ? mA : 1.f;
+          const float sj = uni ? mA : 1.f;
           // a_j gets -(sum m_i d_ij / r^3): d_ij points from i to j
           part[((size_t)A * 4 + c) * N + c0 + l] = c < 3 ? -(s * sj) : s * sj;
         }

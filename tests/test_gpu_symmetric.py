@@ -100,3 +100,62 @@ def test_small_systems_fall_back(ctx):
     acc, _ = _run(ctx, p32, m32, h32, _sym(), phi=False)
     ref, _ = orc.accel(p32, h32, p32, m32, h32, want_phi=False)
     assert err_stats(rel_err(acc, ref))["p99"] <= 1e-5
+
+
+def _uniform_mass_system(n, seed):
+    # every mass 1/n, softening drawn per body: superblocks are flagged
+    # uniform in mass but not in softening
+    rng = np.random.default_rng(seed)
+    p32 = rng.normal(size=(3, n)).astype(np.float32)
+    m32 = np.full(n, 1.0 / n, np.float32)
+    h32 = rng.uniform(5e-6, 5e-5, n).astype(np.float32)
+    return p32, m32, h32
+
+
+@pytest.mark.parametrize("n,tune", [(9000, "sym4"), (65536, "sym8"), (65536, None)])
+def test_uniform_mass_mixed_softening(ctx, n, tune):
+    # off-diagonal items of such superblocks take the generic pair loop, whose
+    # sums already carry m_j (i side) and m_i (j side); a second mass factor
+    # would scale every such contribution by 1/n
+    p32, m32, h32 = _uniform_mass_system(n, seed=n)
+    t = {"sym4": _sym(4), "sym8": _sym(8), None: None}[tune]
+    # tune None: 65536 bodies make 528 items of 2048-body superblocks, so
+    # khrr_accel takes the pair-symmetric path by itself
+    acc, phi = _run(ctx, p32, m32, h32, t)
+    rows = np.random.default_rng(1).choice(n, size=min(n, 4096), replace=False)
+    ref_acc, ref_phi = orc.accel_rows(rows, p32, h32, m32)
+    s = err_stats(rel_err(acc[:, rows], ref_acc))
+    print("uniform mass, mixed softening", n, tune, s)
+    assert s["median"] <= 1e-6 and s["p99"] <= 1e-5, s
+    assert np.max(np.abs(phi[rows] - ref_phi) / np.abs(ref_phi)) <= 2e-6
+
+
+def test_uniform_mass_mixed_softening_batched(ctx):
+    # the same systems through the batched form: each equals its plain
+    # pair-symmetric launch bit for bit and the oracle within the bar
+    import torch
+    import khrr
+    sys_ = [_uniform_mass_system(n, seed=7 + n) for n in (20000, 40000, 3000)]
+    off = np.concatenate([[0], np.cumsum([s[0].shape[1] for s in sys_])]).astype(np.int32)
+    pos = np.concatenate([s[0] for s in sys_], axis=1)
+    m = np.concatenate([s[1] for s in sys_])
+    h = np.concatenate([s[2] for s in sys_])
+    N = pos.shape[1]
+    acc = torch.zeros((3, N), device="cuda:0")
+    ph = torch.zeros(N, device="cuda:0")
+    ctx.accel_batched(to_dev(off, torch.int32), off.tolist(), to_dev(pos, torch.float32),
+                      to_dev(h, torch.float32), to_dev(m, torch.float32), acc=acc, phi=ph)
+    torch.cuda.synchronize()
+    acc, ph = acc.cpu().numpy(), ph.cpu().numpy()
+    for k, (p32, m32, h32) in enumerate(sys_):
+        a, b = off[k], off[k + 1]
+        n = b - a
+        plain = khrr.Tune(8, 4, 1, 256, 1) if n >= 16384 else \
+            khrr.Tune(4, 4, max(1, min(32, -(-n // 1024))), 256, 0)
+        pa, pp = _run(ctx, p32, m32, h32, plain)
+        np.testing.assert_array_equal(acc[:, a:b], pa.astype(np.float32), err_msg=f"system {k}")
+        np.testing.assert_array_equal(ph[a:b], pp.astype(np.float32), err_msg=f"system {k}")
+        rows = np.random.default_rng(k).choice(n, size=min(n, 2048), replace=False)
+        ref_acc, _ = orc.accel_rows(rows, p32, h32, m32)
+        s = err_stats(rel_err(acc[:, a:b][:, rows], ref_acc))
+        assert s["median"] <= 1e-6 and s["p99"] <= 1e-5, (k, s)
