@@ -193,6 +193,35 @@ int round_ipl(const khrr_ctx* ctx, int n_sys, const int* count, int pref);
 // Scratch bytes the batched form needs for the partial sums of one system
 // of n bodies (0 when it writes its outputs directly).
 size_t batched_part_bytes(int n, bool compensated);
+// Binning (khrr_p2mesh.hip).  Scratch of one call of n bodies on G cells,
+// in 256-byte aligned pieces: the cell indices, the radix sort's two key
+// and two value buffers, its tile histograms and digit totals, and the
+// bounds of the 512-cell blocks.  constexpr, so a fixed buffer can be sized
+// from it.
+constexpr int kBinSortTile = 2048;  // bodies per sort tile
+constexpr int kBinRadix = 256;      // digits per sort pass
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t bin_scratch_need(long long n, long long G) {
+  const long long nn = n > 0 ? n : 1;
+  const long long ntiles = n > 0 ? (n + kBinSortTile - 1) / kBinSortTile : 1;
+  const long long cblocks = (G + KHRR_BIN_BLOCK - 1) / KHRR_BIN_BLOCK;
+  return 5 * align256((size_t)nn * 4) + align256((size_t)kBinRadix * ntiles * 4) +
+         align256((size_t)kBinRadix * 4) + align256((size_t)(cblocks + 1) * 4);
+}
+// Argument checks of khrr_p2mesh_bin (the grid via check_grid).
+int bin_validate(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos,
+                 const double* mass, const int* count, const double* cell_mass);
+// Enqueues one binning call with its scratch at `base` (at least
+// bin_scratch_need(n, G) bytes, 256-byte aligned).  timed: bracket it with
+// the context's kernel timers (their events come from a pool that may
+// create one).
+int bin_launch(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos, const double* mass,
+               int* cell_id, int* id_in_cell, int* count, int* offset, int* start_id_ref,
+               int* block_incl, double* cell_mass, void* base, bool timed);
+// Grid shapes khrr_p2mesh_bin accepts (khrr_p2mesh.hip): every size >= 1,
+// at most INT_MAX cells, and the reference's cell index (z clamped by
+// size[1], y stride size[2]) never past the last cell.
+int check_grid(khrr_ctx* ctx, const khrr_grid* grid);
 // Pair-symmetric form (khrr_sym.hip) for one system (i-set == j-set).
 // Bodies per superblock for a system of n bodies: 4096 from 524288 bodies
 // on, else 2048 (a rule of n alone, shared by the plain and batched forms).

@@ -502,20 +502,16 @@ extern "C" int khrr_p2mesh_cell_avg(khrr_ctx* ctx, int n, long long n_cells, con
   return check_launch(ctx, "cell_avg_kernel");
 }
 
-extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos,
-                               const double* mass, int* cell_id, int* id_in_cell, int* count,
-                               int* offset, int* start_id_ref, int* block_incl,
-                               double* cell_mass) {
-  using namespace khrr;
-  KHRR_TRY(enter(ctx));
+namespace khrr {
+
+int check_grid(khrr_ctx* ctx, const khrr_grid* grid) {
   if (!grid) return fail(ctx, KHRR_EINVAL, "grid is NULL");
-  if (n < 0) return fail(ctx, KHRR_EINVAL, "n < 0");
   const long long G = (long long)grid->size[0] * grid->size[1] * grid->size[2];
   if (grid->size[0] <= 0 || grid->size[1] <= 0 || grid->size[2] <= 0 || G > INT_MAX)
     return fail(ctx, KHRR_EINVAL, "invalid grid size %d x %d x %d", grid->size[0], grid->size[1],
                 grid->size[2]);
   // The reference clamps z with size[1] and strides y by size[2]; for a
-  // non-cubic grid that can index past G, which this entry point rejects.
+  // non-cubic grid that can index past G, which is rejected here.
   const long long max_cell = (long long)(grid->size[1] - 1) * grid->size[0] * grid->size[1] +
                              (long long)(grid->size[1] - 1) * grid->size[2] + (grid->size[0] - 1);
   if (max_cell >= G)
@@ -523,18 +519,30 @@ extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, cons
                 "grid %d x %d x %d: the reference's cell index (z clamped by size[1], y "
                 "stride size[2]) can exceed the cell count for this shape",
                 grid->size[0], grid->size[1], grid->size[2]);
+  return 0;
+}
+
+}  // namespace khrr
+
+namespace khrr {
+
+int bin_validate(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos,
+                 const double* mass, const int* count, const double* cell_mass) {
+  KHRR_TRY(check_grid(ctx, grid));
+  if (n < 0) return fail(ctx, KHRR_EINVAL, "n < 0");
   if (!count) return fail(ctx, KHRR_EINVAL, "count is NULL");
   if (n > 0 && !pos) return fail(ctx, KHRR_EINVAL, "pos is NULL");
   if (n > 0 && cell_mass && !mass) return fail(ctx, KHRR_EINVAL, "mass is NULL");
+  return 0;
+}
 
+int bin_launch(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos, const double* mass,
+               int* cell_id, int* id_in_cell, int* count, int* offset, int* start_id_ref,
+               int* block_incl, double* cell_mass, void* base, bool timed) {
+  static_assert(kBinSortTile == kSortTile && kBinRadix == kRadix, "scratch layout constants");
+  const long long G = (long long)grid->size[0] * grid->size[1] * grid->size[2];
   const int ntiles = n > 0 ? (int)ceil_div(n, kSortTile) : 1;
   const int cblocks = (int)ceil_div(G, kCellBlock);
-  const size_t need = 5 * Carver::need((size_t)(n > 0 ? n : 1), sizeof(int)) +
-                      Carver::need((size_t)kRadix * ntiles, sizeof(int)) +
-                      Carver::need((size_t)kRadix, sizeof(int)) +
-                      Carver::need((size_t)cblocks + 1, sizeof(int));
-  void* base = nullptr;
-  KHRR_TRY(scratch(ctx, need, &base));
   Carver cv(base);
   int* cells = cv.take<int>(n > 0 ? n : 1);
   int* ka = cv.take<int>(n > 0 ? n : 1);
@@ -544,6 +552,8 @@ extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, cons
   int* hist = cv.take<int>((size_t)kRadix * ntiles);
   int* totals = cv.take<int>((size_t)kRadix);
   int* bounds = cv.take<int>((size_t)cblocks + 1);
+  if (cv.used > bin_scratch_need(n, G))
+    return fail(ctx, KHRR_EINVAL, "binning scratch layout exceeds its size");
   if (cell_id) cells = cell_id;
 
   GridArgs g{grid->min[0], grid->min[1], grid->min[2], grid->cell[0], grid->cell[1],
@@ -552,7 +562,7 @@ extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, cons
   while (bits < 31 && (1ll << bits) < G) ++bits;
   const int passes = (bits + kRadixBits - 1) / kRadixBits;
 
-  Timed tm(ctx, KHRR_K_BIN);
+  Timed tm(timed ? ctx : nullptr, KHRR_K_BIN);
   const int* skeys = cells;
   const int* svals = nullptr;
   if (n > 0) {
@@ -592,4 +602,20 @@ extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, cons
                      bounds, mass, count, offset, start_id_ref, block_incl, id_in_cell,
                      cell_mass);
   return check_launch(ctx, "p2mesh kernels");
+}
+
+}  // namespace khrr
+
+extern "C" int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos,
+                               const double* mass, int* cell_id, int* id_in_cell, int* count,
+                               int* offset, int* start_id_ref, int* block_incl,
+                               double* cell_mass) {
+  using namespace khrr;
+  KHRR_TRY(enter(ctx));
+  KHRR_TRY(bin_validate(ctx, grid, n, pos, mass, count, cell_mass));
+  const long long G = (long long)grid->size[0] * grid->size[1] * grid->size[2];
+  void* base = nullptr;
+  KHRR_TRY(scratch(ctx, bin_scratch_need(n, G), &base));
+  return bin_launch(ctx, grid, n, pos, mass, cell_id, id_in_cell, count, offset, start_id_ref,
+                    block_incl, cell_mass, base, true);
 }

@@ -244,15 +244,16 @@ int finish(khrr_server* s, const std::vector<Sim*>& done, long long* ids, int ma
                                  hipMemcpyDeviceToHost, ctx->stream));
     KHRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     KHRR_HIP(ctx, hipFree(m->dev));
+    // released: out of the active set at once, so an error later in this
+    // loop cannot leave a freed simulation active
     m->dev = nullptr;
     s->ranges.give(m->begin, m->n);
     s->in_use -= footprint(m->n);
     m->state = State::kFinished;
+    s->active.erase(std::find(s->active.begin(), s->active.end(), m));
     if (*n_ids < max_ids && ids) ids[*n_ids] = m->id;
     ++*n_ids;
   }
-  auto is_done = [](Sim* m) { return m->state == State::kFinished; };
-  s->active.erase(std::remove_if(s->active.begin(), s->active.end(), is_done), s->active.end());
   return 0;
 }
 
@@ -344,10 +345,9 @@ int khrr_server_submit(khrr_server* s, const khrr_request* req, long long* id) {
   if (req->n < 1 || !req->pos || !req->vel || !req->mass || !req->eps)
     return khrr::fail(ctx, KHRR_EINVAL, "request needs n >= 1 and host arrays");
   if (req->steps < 0 || !(req->dt > 0.0)) return khrr::fail(ctx, KHRR_EINVAL, "bad dt/steps");
-  const khrr_grid& g = req->grid;
-  const long long G = (long long)g.size[0] * g.size[1] * g.size[2];
-  if (g.size[0] < 1 || g.size[1] < 1 || g.size[2] < 1 || G > INT32_MAX)
-    return khrr::fail(ctx, KHRR_EINVAL, "bad result grid");
+  // the result grid is binned when the simulation finishes: refuse now what
+  // khrr_p2mesh_bin would refuse then
+  KHRR_TRY(khrr::check_grid(ctx, &req->grid));
   if (req->n > s->cap || footprint(req->n) > s->budget)
     return khrr::fail(ctx, KHRR_ENOMEM, "request of %d bodies can never fit the budget", req->n);
   auto m = std::make_unique<Sim>();

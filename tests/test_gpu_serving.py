@@ -2,6 +2,8 @@
 device-memory footprint.  Each result must be bit-identical whether the
 request ran alone or among others; energies follow the FP64 oracle loop
 (nbody.cu:1468-1594) within FP32-force tolerance (1e-5 relative)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -135,6 +137,20 @@ def test_errors(ctx):
         srv.result(rid, 16 ** 3)
     with pytest.raises(khrr.KhrrError, match="unknown request"):
         srv.result(12345, 16 ** 3)
+    # a grid khrr_p2mesh_bin refuses (16 x 32 x 16: the reference's cell
+    # index, z clamped by size[1] and y strided by size[2], can pass the last
+    # cell) is refused at submit, not when the simulation finishes
+    bad = khrr.Grid((ctypes.c_double * 3)(-2.0, -2.0, -2.0), (ctypes.c_double * 3)(0.25, 0.125, 0.25),
+                    (ctypes.c_int * 3)(16, 32, 16))
+    with pytest.raises(khrr.KhrrError, match="can exceed the cell count"):
+        srv.submit(*small, 0.002, 3, bad)
+    assert srv.stats()["pending"] == 1
+    # the queued request still runs to the end and is collected
+    for _ in range(20):
+        if srv.step():
+            break
+    d0, d1, cm = srv.result(rid, 16 ** 3)
+    assert cm.sum() == pytest.approx(1.0)
     srv.close()
 
 
@@ -170,3 +186,63 @@ def test_round_pairs_rejects_negative(ctx):
     with pytest.raises(khrr.KhrrError):
         srv.set_round_pairs(-1.0)
     srv.close()
+
+
+def test_c4_sized_requests_alone_equal_mix(ctx):
+    """The serving workload's request shapes at full size: a disk + halo
+    galaxy of 65536 + 131072 bodies and a 131072-body Plummer sphere among
+    small spheres, each returning its FP64 mass on a 128^3 grid over
+    [-2, 2]^3.  Each large result equals the same request run alone, bit for
+    bit; every grid holds its request's mass; a small request's energy at
+    t = 0 and at the end follows the FP64 oracle loop."""
+    import khrr
+    grid = khrr.cube_grid(128, -2.0, 2.0)
+    G = 128 ** 3
+
+    def run(srv, reqs, arrivals):
+        ids, done, rnd = {}, {}, 0
+        while len(done) < len(reqs):
+            for k, r in enumerate(reqs):
+                if k not in ids and arrivals[k] <= rnd:
+                    ids[k] = srv.submit(*r["arrays"], 0.002, r["steps"], grid)
+            for fid in srv.step():
+                k = next(k for k, v in ids.items() if v == fid)
+                done[k] = srv.result(fid, G)
+            rnd += 1
+            assert rnd < 1000
+        return done
+
+    galaxy = dict(arrays=_galaxy_req(65536, 131072, 200001), steps=8)
+    sphere = dict(arrays=_plummer_req(131072, 200002), steps=6)
+    small = dict(arrays=_plummer_req(2048, 200003), steps=10)
+    others = [dict(arrays=_plummer_req(n, 200010 + n), steps=s)
+              for n, s in ((1024, 12), (5120, 7), (20480, 5))]
+    budget = 8 << 30
+    alone = {}
+    for name, r in (("galaxy", galaxy), ("sphere", sphere)):
+        srv = khrr.Server(ctx, budget, 64)
+        alone[name] = run(srv, [r], [0])[0]
+        srv.close()
+    srv = khrr.Server(ctx, budget, 64)
+    reqs = [others[0], galaxy, small, sphere, others[1], others[2]]
+    mix = run(srv, reqs, [0, 0, 1, 2, 3, 4])
+    srv.close()
+    for name, k in (("galaxy", 1), ("sphere", 3)):
+        a, b = alone[name], mix[k]
+        assert a[0] == b[0] and a[1] == b[1], name  # diagnostics, exact
+        np.testing.assert_array_equal(a[2], b[2], err_msg=name)
+    # every grid holds its request's FP64 mass (bodies outside the box sit
+    # in the edge cells)
+    for k, r in enumerate(reqs):
+        M = r["arrays"][2].sum()
+        assert abs(mix[k][2].sum() - M) <= 1e-12 * M, k
+        assert mix[k][2].shape == (G,)
+    assert abs(mix[1][2].sum() - (1.0 + 6.699356)) < 1e-9
+    # the small request's energy against the oracle loop
+    pos, vel, mass, eps = small["arrays"]
+    h = (0.5 * eps * eps).astype(np.float32).astype(np.float64)
+    m32 = mass.astype(np.float32).astype(np.float64)
+    _, _, d = orc.run(pos, vel, m32, h, 0.002, small["steps"])
+    E = lambda dd: 0.5 * (dd["Ek"] + dd["Ep"])  # noqa: E731
+    assert abs(E(mix[2][0]) / (0.5 * (d[0, 0] + d[0, 1])) - 1) < 1e-5
+    assert abs(E(mix[2][1]) / (0.5 * (d[-1, 0] + d[-1, 1])) - 1) < 1e-5
