@@ -156,8 +156,18 @@ bool load_restart(const std::string& bin, int i_cont, const std::vector<GalaxyPr
       return false;
     }
     int cnt = 0;
-    bool ok = std::fread(&cnt, sizeof(int), 1, f) == 1 && std::fread(&t, sizeof(double), 1, f) == 1;
-    ok = ok && cnt == count;
+    if (std::fread(&cnt, sizeof(int), 1, f) != 1 || std::fread(&t, sizeof(double), 1, f) != 1) {
+      std::fclose(f);
+      err = "short snapshot header in " + path;
+      return false;
+    }
+    if (cnt != count) {
+      std::fclose(f);
+      err = path + " holds " + std::to_string(cnt) + " bodies, the galaxies have " +
+            std::to_string(count);
+      return false;
+    }
+    bool ok = true;
     const size_t N = (size_t)p.n;
     for (int i = i0; ok && i < i0 + count; ++i) {
       double r[6];
@@ -168,7 +178,7 @@ bool load_restart(const std::string& bin, int i_cont, const std::vector<GalaxyPr
       }
     }
     std::fclose(f);
-    if (!ok) err = "bad or short snapshot " + path;
+    if (!ok) err = "short snapshot " + path + " (fewer than " + std::to_string(count) + " bodies)";
     return ok;
   };
   int ns = 0, nd = 0;

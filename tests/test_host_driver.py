@@ -221,3 +221,88 @@ def test_debug_dumps(tmp_path):
     np.testing.assert_allclose(float(out["total mpart"]), mass.sum(), rtol=1e-5)
     np.testing.assert_allclose(float(out["nbody part acc sum"]), pabs.sum(), rtol=1e-4)
     np.testing.assert_allclose(float(out["cell part acc sum"]), cpabs.sum(), rtol=1e-4)
+
+
+# ---- restart (i_cont > 0), nbody.cu:1247-1314 ----------------------------
+
+def _set_start(base, i_cont, tmax, dtsave):
+    """__start_nbody.ini in the reference's layout (nbody.cu:961-984)."""
+    with open(os.path.join(base, "ini", "__start_nbody.ini"), "w") as f:
+        f.write(f"{i_cont} \t--- i_cont\n{tmax} \t--- tmax\n{dtsave} \t--- dt_save\n")
+
+
+def _snapshot(path, count, t, records=None):
+    """S_/DM_ .bin layout (nbody.cu:726-755): int count, double t, then
+    x, y, z, vx, vy, vz per body."""
+    records = count if records is None else records
+    vals = np.random.default_rng(abs(count) + 1).normal(size=6 * records)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<i", count) + struct.pack("<d", t) + vals.astype("<f8").tobytes())
+
+
+@pytest.mark.parametrize("case,message", [
+    ("no_stars", "S_   10.bin"),
+    ("short_stars", "short snapshot"),
+    ("wrong_count", "holds 63 bodies, the galaxies have 64"),
+    ("short_header", "short snapshot header"),
+    ("no_dm", "DM_   10.bin"),
+    ("no_lie", "LIE_0.bin"),
+    ("short_lie", "short LIE_0.bin"),
+])
+def test_restart_input_errors(tmp_path, case, message):
+    # every failure is found while reading, before the device is touched
+    base = _layout(str(tmp_path), [dict(n_s=64, n_dm=32, seed=9)], 0.04, 0.002)
+    _set_start(base, 10, 0.04, 0.002)
+    b = os.path.join(base, "bin")
+    s_path, d_path = os.path.join(b, "S_   10.bin"), os.path.join(b, "DM_   10.bin")
+    if case != "no_stars":
+        if case == "short_stars":
+            _snapshot(s_path, 64, 0.02, records=10)
+        elif case == "wrong_count":
+            _snapshot(s_path, 63, 0.02)
+        elif case == "short_header":
+            open(s_path, "wb").write(struct.pack("<i", 64))
+        else:
+            _snapshot(s_path, 64, 0.02)
+    if case not in ("no_stars", "short_stars", "wrong_count", "short_header", "no_dm"):
+        _snapshot(d_path, 32, 0.02)
+    if case == "short_lie":
+        open(os.path.join(b, "LIE_0.bin"), "wb").write(b"\0" * 40)
+    r = subprocess.run([BIN, "--data", base], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1, (case, r.stdout[-500:], r.stderr[-500:])
+    assert message in r.stderr, (case, r.stderr)
+
+
+@pytest.mark.gpu
+def test_restart_continues_bit_for_bit(tmp_path):
+    # 20 saves straight through against 10 saves then a restart from
+    # i_cont = 10 to save 20: the snapshots, LIE_0.bin and every CSV row
+    # after the restart are byte-identical (the restart recomputes acc from
+    # the snapshot's positions, which are the positions the straight run's
+    # last force was evaluated at; --clear-cell-phi because the reference's
+    # never-cleared cell-potential buffer does not survive a restart)
+    gal = [dict(n_s=1024, n_dm=2048, seed=11)]
+    a = _layout(str(tmp_path / "a"), gal, 0.04, 0.002)
+    b = _layout(str(tmp_path / "b"), gal, 0.02, 0.002)
+    args = ["--grid", "32", "0.125", "--clear-cell-phi"]
+    for base in (a, b):
+        r = subprocess.run([BIN, "--data", base] + args, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+    assert os.path.exists(os.path.join(b, "bin", "S_   10.bin"))
+    assert not os.path.exists(os.path.join(b, "bin", "S_   20.bin"))
+    _set_start(b, 10, 0.04, 0.002)
+    r = subprocess.run([BIN, "--data", b] + args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    files = ["bin/S_   10.bin", "bin/DM_   10.bin", "bin/S_   20.bin", "bin/DM_   20.bin",
+             "bin/LIE_0.bin", "out/E(t).csv", "out/Imp(t).csv", "out/LL(t).csv", "out/Lz(t).csv",
+             "out/sample.csv", "out/phi/phi_   11.csv", "out/phi/phi_   20.csv"]
+    for f in files:
+        fa = open(os.path.join(a, f), "rb").read()
+        fb = open(os.path.join(b, f), "rb").read()
+        assert fa == fb, f
+    E = np.genfromtxt(os.path.join(b, "out", "E(t).csv"), delimiter=",", skip_header=1)
+    np.testing.assert_array_equal(E[:, 0], np.arange(21))
+    raw = open(os.path.join(b, "bin", "S_   20.bin"), "rb").read()
+    assert struct.unpack("<i", raw[:4])[0] == 1024
+    assert abs(struct.unpack("<d", raw[4:12])[0] - 0.04) < 1e-12
