@@ -10,8 +10,8 @@ CPU:
   the source, so doubling the mass doubles the field bit for bit over the
   first 10 iterations and to 1e-6 of its maximum over 200 (by then the
   leading fronts have passed through the FP32 subnormal range, where
-  rounding does not scale), and 20
-  iterations against the FP64 oracle (max-norm relative error <= 1e-5);
+  rounding does not scale), and the 200 iterations against the FP64
+  oracle (max-norm relative error <= 1e-5);
 * the batched form on the full 512-system composition (13.9M bodies, N
   log-uniform in [1024, 131072]): systems across the size range equal their
   plain single-system launches bit for bit;
@@ -88,10 +88,12 @@ def test_headline_stencil(ctx, galaxy):
     f1 = run(cm, 200).astype(np.float64)
     f2 = run(2.0 * cm, 200).astype(np.float64)
     assert np.abs(f2 - 2.0 * f1).max() <= 1e-6 * np.abs(2.0 * f1).max()
-    f20 = run(cm, 20).astype(np.float64)
+    # the headline's 200 iterations against the FP64 oracle (OpenMP, a few
+    # seconds on the host): max-norm relative error <= 1e-5
     ref, _ = orc.wave_iterate(orc.reference_wave(nx, 4.0 / nx), np.zeros(G), np.zeros(G),
-                              cm.cpu().numpy(), 20)
-    err = np.abs(f20 - ref).max() / np.abs(ref).max()
+                              cm.cpu().numpy(), 200)
+    err = np.abs(f1 - ref).max() / np.abs(ref).max()
+    print(f"stencil 256^3 x 200 iterations: FP32 vs FP64 max-norm relative error {err:.3e}")
     assert err <= 1e-5, err
 
 
