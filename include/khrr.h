@@ -245,6 +245,20 @@ int khrr_p2mesh_bin(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* p
                     const double* mass, int* cell_id, int* id_in_cell, int* count, int* offset,
                     int* start_id_ref, int* block_incl, double* cell_mass);
 
+/* Allocation-free form of khrr_p2mesh_bin: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops): same arguments, kernels and
+ * results as khrr_p2mesh_bin, but the scratch is one fixed device array of
+ * the library (about 21 MB per device, held from the first context on that
+ * device until the process ends) instead of the context's growing arena,
+ * and no timers are recorded.  Limits: n <= 1048576 bodies and at most
+ * 16777216 (256^3) cells; larger calls fail with KHRR_EINVAL.  The array
+ * is shared by every call on one device: calls that overlap on one device
+ * race and must be serialised by the caller.  Calls no hipMalloc/hipFree. */
+int khrr_p2mesh_bin_noalloc(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos,
+                            const double* mass, int* cell_id, int* id_in_cell, int* count,
+                            int* offset, int* start_id_ref, int* block_incl, double* cell_mass);
+
 /* computeCellPhiUnsorted (p2mesh-kernel.cuh:152-167): for every particle i
  * of cell c, cell_phi[c] += phi[i] / count[c], summed in FP64 in particle
  * index order (the reference's FP64 atomics have no fixed order).  Uses

@@ -147,6 +147,13 @@ int khrr_ctx_create(int device, khrr_ctx** out) {
     delete ctx;
     return khrr::fail(nullptr, KHRR_ENOMEM, "pinned staging allocation failed");
   }
+  if (khrr::noalloc_init(ctx) != 0) {
+    const std::string msg = ctx->err;
+    (void)hipHostFree(ctx->host_stage);
+    (void)hipFree(ctx->dev_words);
+    delete ctx;
+    return khrr::fail(nullptr, KHRR_EDEVICE, "%s", msg.c_str());
+  }
   *out = ctx;
   return 0;
 }

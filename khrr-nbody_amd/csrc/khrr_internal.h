@@ -218,6 +218,10 @@ int bin_validate(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos,
 int bin_launch(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos, const double* mass,
                int* cell_id, int* id_in_cell, int* count, int* offset, int* start_id_ref,
                int* block_incl, double* cell_mass, void* base, bool timed);
+// Looks up, once per device, the device address of the fixed scratch of
+// khrr_p2mesh_bin_noalloc (khrr_noalloc.hip); called when a context is
+// created, with its device current.
+int noalloc_init(khrr_ctx* ctx);
 // Grid shapes khrr_p2mesh_bin accepts (khrr_p2mesh.hip): every size >= 1,
 // at most INT_MAX cells, and the reference's cell index (z clamped by
 // size[1], y stride size[2]) never past the last cell.

@@ -116,6 +116,8 @@ _SIGS = {
     "khrr_diag": (_I, [_P, _I, _P, _P, _P, _P, ctypes.POINTER(_D)]),
     "khrr_p2mesh_bin": (_I, [_P, ctypes.POINTER(Grid), _I, _P, _P, _P, _P, _P, _P, _P, _P,
                              _P]),
+    "khrr_p2mesh_bin_noalloc": (_I, [_P, ctypes.POINTER(Grid), _I, _P, _P, _P, _P, _P, _P, _P,
+                                     _P, _P]),
     "khrr_p2mesh_cell_phi": (_I, [_P, _I, ctypes.c_longlong, _P, _P, _P, _P, _P, _P, _I]),
     "khrr_mesh_acc": (_I, [_P, _I, _D, _P, _P, _P]),
     "khrr_mesh_to_particles": (_I, [_P, _I, ctypes.c_longlong, _P, _P, _P, _P, _P]),
@@ -349,6 +351,24 @@ class Context:
         B = (G + BIN_BLOCK - 1) // BIN_BLOCK
         i32 = torch.int32
         self._check(self._lib.khrr_p2mesh_bin(
+            self._h, ctypes.byref(grid), n, _ptr(pos, torch.float64, (3, n), "pos"),
+            _ptr(mass, torch.float64, (n,), "mass"), _ptr(cell_id, i32, (n,), "cell_id"),
+            _ptr(id_in_cell, i32, (n,), "id_in_cell"), _ptr(count, i32, (G,), "count"),
+            _ptr(offset, i32, (G,), "offset"), _ptr(start_id_ref, i32, (G,), "start_id_ref"),
+            _ptr(block_incl, i32, (B,), "block_incl"),
+            _ptr(cell_mass, torch.float64, (G,), "cell_mass")))
+
+    def p2mesh_bin_noalloc(self, grid: Grid, pos, mass, cell_id=None, id_in_cell=None,
+                           count=None, offset=None, start_id_ref=None, block_incl=None,
+                           cell_mass=None):
+        """p2mesh_bin with the library's fixed scratch (allocates nothing;
+        at most 1048576 bodies on 256^3 cells)."""
+        import torch
+        n = pos.shape[1]
+        G = grid.size[0] * grid.size[1] * grid.size[2]
+        B = (G + BIN_BLOCK - 1) // BIN_BLOCK
+        i32 = torch.int32
+        self._check(self._lib.khrr_p2mesh_bin_noalloc(
             self._h, ctypes.byref(grid), n, _ptr(pos, torch.float64, (3, n), "pos"),
             _ptr(mass, torch.float64, (n,), "mass"), _ptr(cell_id, i32, (n,), "cell_id"),
             _ptr(id_in_cell, i32, (n,), "id_in_cell"), _ptr(count, i32, (G,), "count"),
