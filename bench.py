@@ -48,6 +48,7 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 FLOP_PER_PAIR = 20
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+TUNE_CACHE = os.path.join(ROOT, "khrr-nbody_amd", "tune_cache.json")
 
 
 def log(msg):
@@ -67,9 +68,10 @@ def parse():
     ap.add_argument("--batched-steps", type=int, default=2)
     ap.add_argument("--batched-run-steps", type=int, default=1000,
                     help="steps of the batched-run leg (512 systems advanced together)")
-    ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--serve-lambda", type=float, default=3.0, help="arrivals per second")
-    ap.add_argument("--serve-tmax", default="0.05,0.5",
+    ap.add_argument("--serve-requests", type=int, default=16,
+                    help="requests of the serving leg (the full stream is 5000)")
+    ap.add_argument("--serve-lambda", type=float, default=1.2, help="arrivals per second")
+    ap.add_argument("--serve-tmax", default="0.2,2.0",
                     help="range of the requests' integration times tmax (uniform), lo,hi")
     ap.add_argument("--serve-saturated", type=float, default=50.0,
                     help="arrival rate of the saturated serving run (0: skip it)")
@@ -79,8 +81,15 @@ def parse():
     ap.add_argument("--precision-steps", type=int, default=3,
                     help="steps of the 3.1M-body encounter per sum mode (0: skip)")
     ap.add_argument("--shape", default=None,
-                    help="fixed launch shape i_per_lane,waves_per_block,j_split,j_tile,symmetric "
-                         "instead of the autotuner (for profiler passes)")
+                    help="fixed launch shape i_per_lane,waves_per_block,j_split,j_tile,symmetric"
+                         "[,superblock] instead of the autotuner (for profiler passes)")
+    ap.add_argument("--tune-cache", default=TUNE_CACHE,
+                    help="autotuner cache (JSON) loaded before tuning; shapes it holds for "
+                         "this device are used as they are ('' = none)")
+    ap.add_argument("--retune", action="store_true",
+                    help="time the candidate shapes again instead of using the cache")
+    ap.add_argument("--tune-save", default=None,
+                    help="write the context's tuned shapes to this JSON file at the end")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 512 MiB device copy first (counter calibration under rocprof)")
     return ap.parse_args()
@@ -316,7 +325,7 @@ def bench_mesh(ctx, sim, bins, field, reps=10):
     return {"grid": "256^3 over [-2,2]^3", "n": n, "entry_points": out}
 
 
-def bench_batched(ctx, steps):
+def bench_batched(ctx, steps, retune=False):
     """512 Plummer spheres: one persistent batched launch per step vs one
     plain launch per system per step (same integrator calls)."""
     import numpy as np
@@ -339,7 +348,12 @@ def bench_batched(ctx, steps):
     off_dev = torch.tensor(off, device="cuda:0", dtype=torch.int32)
     off_list = off.tolist()
     pairs = float(np.sum(sizes.astype(np.float64) ** 2))
-    tune, _ = ctx.tune_batched(off_list)
+    tune = None if retune else ctx.tune_batched_cached()
+    tune_source = "cache"
+    if tune is None:
+        tune, _ = ctx.tune_batched(off_list)
+        tune_source = "tuned"
+    log(f"batched launch shape {tune.as_dict()} ({tune_source})")
 
     def grav_batched(out):
         # same composition and buffers every step: captured once, then replayed
@@ -397,7 +411,8 @@ def bench_batched(ctx, steps):
     return {"systems": 512, "bodies": N, "pairs_per_step": pairs,
             "batched_ms": t_b * 1e3, "plain_ms": t_p * 1e3,
             "batched_Gpairs_per_s": pairs / t_b / 1e9, "plain_Gpairs_per_s": pairs / t_p / 1e9,
-            "speedup_vs_plain": t_p / t_b, "tune": tune.as_dict(), "graph": True,
+            "speedup_vs_plain": t_p / t_b, "tune": tune.as_dict(), "tune_source": tune_source,
+            "graph": True,
             "plain": "khrr_accel per system, default launch shape, one stream",
             "plain_one_sided_ms": t_o * 1e3,
             "speedup_vs_plain_one_sided": t_o / t_b,
@@ -453,7 +468,7 @@ def bench_batched_run(ctx, steps, budget_gb=32.0):
                     "mass 1 per system, so dP and dL are absolute"}
 
 
-def serving_requests(n_req, seed=7, tmax_lo=0.05, tmax_hi=0.5, dt=0.002):
+def serving_requests(n_req, seed=7, tmax_lo=0.2, tmax_hi=2.0, dt=0.002):
     """Mixed request stream: 90 % Plummer spheres (N log-uniform on
     [1024, 131072], multiple of 1024, eps 0.004 (N/16384)^-1/3), 10 % disk +
     halo galaxies (65536 + 131072); tmax uniform on [tmax_lo, tmax_hi]."""
@@ -485,7 +500,7 @@ def poisson_arrivals(n_req, lam, seed=8):
 
 
 def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4e10,
-                  lam_saturated=50.0, tmax=(0.05, 0.5)):
+                  lam_saturated=50.0, tmax=(0.2, 2.0)):
     """Continuous batching on a Poisson arrival stream vs the same requests
     run one at a time (FIFO) with plain per-system launches; then the same
     requests arriving faster than either can serve them (lam_saturated)."""
@@ -623,36 +638,124 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4
     return out
 
 
-def cpu_leg(checks, rows_n, result):
-    """The CPU leg: the FP64 oracle (OpenMP, all host cores) timed on a
-    bounded sample of the headline workload -- the reported CPU baseline --
-    and, as the checker of the GPU legs, the FP64 references behind their
-    error figures: headline forces at the final positions, the stencil
-    field after 200 iterations, the precision form's forces at t = 0."""
+def host_cpu_info():
+    """CPU model, physical cores and logical CPUs of this host from
+    /proc/cpuinfo (physical cores: distinct (physical id, core id) pairs)."""
+    model, cores, logical = "", set(), 0
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                key, _, val = ln.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "processor":
+                    logical += 1
+                elif key == "model name" and not model:
+                    model = val
+                elif key == "physical id":
+                    phys = val
+                elif key == "core id":
+                    core = val
+                elif not key and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+            if phys is not None:
+                cores.add((phys, core))
+    except OSError:
+        pass
+    return {"cpu_model": model, "physical_cores": len(cores) or None,
+            "logical_cpus": logical or None, "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def median_time(fn, reps=3):
+    """Median wall time of reps calls of fn (and every run's time)."""
     import numpy as np
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_leg(checks, rows_n, result):
+    """The CPU leg: the FP64 oracle (OpenMP, the host cores given to this
+    process) timed on bounded samples of the headline workload -- the
+    reported CPU baseline, median of 3 runs each: gravity rows, binning on
+    the headline grid, stencil iterations and the 24576-body galaxy's step
+    loop -- and, as the checker of the GPU legs, the FP64 references behind
+    their error figures: headline forces at the final positions, the
+    stencil field after 200 iterations, the precision form's forces."""
+    import numpy as np
+    import ics
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     hd = checks["headline"]
     n = hd["pos"].shape[1]
     rows = np.sort(np.random.default_rng(4096).choice(n, rows_n, replace=False))
-    t0 = time.perf_counter()
-    ref, _ = orc.accel_rows(rows, hd["pos"], hd["h"], hd["m"])
-    dt = time.perf_counter() - t0
+    out = {}
+
+    def gravity_rows():
+        out["ref"], _ = orc.accel_rows(rows, hd["pos"], hd["h"], hd["m"])
+
+    dt, dts = median_time(gravity_rows)
+    ref = out["ref"]
     e = np.linalg.norm(hd["acc"][:, rows] - ref, axis=0) / np.linalg.norm(ref, axis=0)
     result["accuracy"] = {"rows": int(rows_n), "median": float(np.median(e)),
                           "p99": float(np.percentile(e, 99)), "max": float(np.max(e)),
                           "reference": "FP64 oracle on the same FP32 inputs"}
+    threads = orc.num_threads()
     cpu = {"value": rows_n * n / dt / 1e9, "unit": "G pair-interactions/s",
-           "cores": orc.num_threads(), "kind": "port",
-           "sample": f"{rows_n} random i-rows x {n} j (one force evaluation subsample, "
-                     f"FP64, OpenMP), {dt:.2f} s"}
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
-                         "")
-        cpu["cpu_model"] = model
-    except OSError:
-        pass
+           "cores": threads, "kind": "port",
+           "sample": f"{rows_n} random i-rows x {n} j of the headline galaxy (a force "
+                     f"evaluation subsample, FP64, OpenMP {threads} threads), median of 3 "
+                     f"runs: {dt:.2f} s",
+           "gravity_runs_s": dts}
+    cpu.update(host_cpu_info())
+    log(f"cpu leg: gravity {cpu['value']:.2f} G pairs/s on {threads} threads")
+    # binning of the headline galaxy's final state on 256^3 over [-2,2]^3
+    bn = checks.get("binning")
+    if bn is not None:
+        nx = bn["nx"]
+
+        def bin_oracle():
+            out["bins"] = orc.p2mesh_bin([-2.0] * 3, [4.0 / nx] * 3, [nx] * 3, bn["pos"],
+                                         bn["mass"])
+
+        tb, tbs = median_time(bin_oracle)
+        exact = all(np.array_equal(bn["out"][k], out["bins"][k]) for k in bn["out"])
+        result["binning"]["bit_exact_vs_oracle"] = bool(exact)
+        log(f"cpu leg: binning outputs bit-exact vs the oracle: {exact}")
+        cpu["binning_ms"] = tb * 1e3
+        cpu["binning_runs_ms"] = [t * 1e3 for t in tbs]
+        cpu["binning_sample"] = (f"{bn['pos'].shape[1]} bodies on {nx}^3 cells, every output; "
+                                 "serial (index-order ranks and sums), median of 3")
+        log(f"cpu leg: binning {tb * 1e3:.1f} ms")
+    # stencil iterations on the headline grid (OpenMP)
+    st = checks.get("stencil")
+    if st is not None:
+        nx, k = st["nx"], 10
+        G = nx ** 3
+        w = orc.reference_wave(nx, 4.0 / nx)
+        ts, tss = median_time(lambda: orc.wave_iterate(w, np.zeros(G), np.zeros(G),
+                                                       st["cell_mass"], k))
+        cpu["stencil_ms_per_iter"] = ts / k * 1e3
+        cpu["stencil_runs_ms_per_iter"] = [t / k * 1e3 for t in tss]
+        cpu["stencil_sample"] = (f"{nx}^3, {k} iterations per run from phi = 0, FP64, OpenMP "
+                                 f"{threads} threads, median of 3")
+        log(f"cpu leg: stencil {ts / k * 1e3:.1f} ms per iteration")
+    # the 8192 + 16384-body galaxy's predictor -> ACCEL -> corrector loop
+    ps, vs, pd, vd = ics.galaxy(8192, 16384, seed=12345)
+    p0, v0, m0, h0 = ics.system_arrays(ps, vs, pd, vd, 1.0, 6.699356, 0.004, 0.004)
+    h0 = h0.astype(np.float32).astype(np.float64)
+    c0_steps = 25
+    tc, tcs = median_time(lambda: orc.run(p0, v0, m0, h0, 0.002, c0_steps, with_diag=False))
+    cpu["c0_500_steps_s"] = tc / c0_steps * 500
+    cpu["c0_runs_s"] = tcs
+    cpu["c0_sample"] = (f"24576-body galaxy, {c0_steps} predictor-ACCEL-corrector steps per "
+                        f"run (FP64, OpenMP {threads} threads), median of 3, scaled to 500 "
+                        "steps (every step costs the same)")
+    log(f"cpu leg: 24576-body galaxy {tc / c0_steps * 1e3:.0f} ms per step")
     result["cpu_baseline"] = cpu
     st = checks.get("stencil")
     if st and "stencil" in result:
@@ -705,12 +808,25 @@ def main():
     ctx = khrr.Context(0)
     # launch shape from the project's autotuner (cached in the context for
     # this size bucket; every later khrr_accel of this size uses it)
+    tune_source = "fixed"
+    if args.tune_cache and os.path.exists(args.tune_cache) and not args.retune:
+        try:
+            ctx.tune_load(args.tune_cache)
+            log(f"tune cache {os.path.relpath(args.tune_cache, ROOT)} loaded")
+        except khrr.KhrrError as e:
+            log(f"tune cache not used: {e}")
     if args.shape:
         tune = khrr.Tune(*[int(v) for v in args.shape.split(",")])
         log(f"fixed launch shape {tune.as_dict()}")
     else:
-        tune, tune_ms = ctx.tune_gravity(n)
-        log(f"autotuned launch shape {tune.as_dict()} ({tune_ms:.2f} ms per trial)")
+        tune = None if args.retune else ctx.tune_cached(n)
+        tune_source = "cache"
+        if tune is None:
+            tune, tune_ms = ctx.tune_gravity(n)
+            tune_source = "tuned"
+            log(f"autotuned launch shape {tune.as_dict()} ({tune_ms:.2f} ms per trial)")
+        else:
+            log(f"launch shape {tune.as_dict()} from the tune cache")
     sim = Sim(ctx, pos, vel, mass, h, 0.002)
     sim.tune = tune
     sim.init_forces()
@@ -745,8 +861,8 @@ def main():
     achieved = FLOP_PER_PAIR * pairs / (grav_avg * 1e-3) / 1e12
     if tune.symmetric:
         # off-diagonal superblock pairs once, diagonal superblocks one-sided
-        # (superblock size by the library's rule of n)
-        sb = 4096 if n >= 524288 else (2048 if n >= 32768 else 1024)
+        # (the shape's superblock size, else the library's rule of n)
+        sb = tune.superblock or (4096 if n >= 524288 else (2048 if n >= 32768 else 1024))
         sizes = [min(sb, n - a) for a in range(0, n, sb)]
         diag_pairs = float(sum(s * s for s in sizes))
         executed = (pairs - diag_pairs) / 2 + diag_pairs
@@ -801,7 +917,7 @@ def main():
         "config": {"workload": "disk+halo galaxy, 262144 stars + 524288 halo, eps 0.004, "
                                "dt 0.002, FP32 direct sum, FP64 predictor-corrector state",
                    "n_bodies": n, "dt": 0.002, "seed": 12345,
-                   "launch_shape": tune.as_dict()},
+                   "launch_shape": tune.as_dict(), "tune_source": tune_source},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic,
@@ -845,6 +961,10 @@ def main():
         log("binning")
         bins, binning = bench_binning(ctx, sim)
         result["binning"] = binning
+        if not args.no_cpu:  # the CPU leg times the oracle on it and checks the outputs
+            checks["binning"] = dict(nx=256, pos=sim.pos.cpu().numpy(),
+                                     mass=sim.mass64.cpu().numpy(),
+                                     out={k: v.cpu().numpy() for k, v in bins.items()})
         if legs & {"stencil", "mesh"}:
             log("stencil")
             checks["stencil"] = {}
@@ -857,7 +977,7 @@ def main():
         torch.cuda.empty_cache()
     if "batched" in legs:
         log("batched")
-        result["batched"] = bench_batched(ctx, args.batched_steps)
+        result["batched"] = bench_batched(ctx, args.batched_steps, retune=args.retune)
         torch.cuda.empty_cache()
     if "batched-run" in legs:
         log(f"batched run: 512 systems, {args.batched_run_steps} steps")
@@ -876,6 +996,9 @@ def main():
     if not args.no_cpu:
         log(f"cpu leg: oracle on {args.cpu_rows} rows of the headline workload")
         cpu_leg(checks, args.cpu_rows, result)
+    if args.tune_save:
+        ctx.tune_save(args.tune_save)
+        log(f"tuned shapes written to {args.tune_save}")
     print(json.dumps(result), flush=True)
 
 

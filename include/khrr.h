@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define KHRR_ABI_VERSION 1
+#define KHRR_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------- */
 #define KHRR_OK 0
@@ -109,10 +109,12 @@ typedef struct {
   int symmetric;    /* 1: when the i-set and j-set are the same buffers,
                        evaluate each unordered pair once and apply it to
                        both bodies (needs i_per_lane 4 or 8 and a partial
-                       buffer of 16 B per body per superblock -- 4096
-                       bodies from n = 524288 on, 2048 from 32768, 1024
-                       below, which always runs 4 i per lane -- at most
+                       buffer of 16 B per body per superblock, at most
                        40 GiB; otherwise the one-sided kernel runs)      */
+  int superblock;   /* pair-symmetric bodies per superblock: 1024, 2048
+                       or 4096 (1024 always runs 4 i per lane), or 0 for
+                       the rule of n: 4096 from n = 524288 on, 2048 from
+                       32768, 1024 below.  Added in ABI version 2.       */
 } khrr_tune;
 
 /* Softened direct sum.  For every i in [0,n_i):
@@ -156,18 +158,36 @@ int khrr_accel_batched(khrr_ctx* ctx, int n_sys, const int* sys_off, const int* 
 
 /* Project-written autotuner: times candidate launch shapes of khrr_accel
  * on synthetic bodies of size n on this context's device (synchronous) and
- * returns the fastest.  Results are cached per power-of-two size bucket in
- * the context; khrr_accel with tune == NULL uses the cache (or a static
- * heuristic when the bucket was never tuned). */
+ * returns the fastest: one-sided i_per_lane x waves_per_block x j_split
+ * with 256-body j-tiles, then j_tile 128/512/1024 for the fastest of
+ * those; pair-symmetric i_per_lane 4/8 x superblock 1024/2048/4096,
+ * compared per pair-interaction.  Results are cached per power-of-two size
+ * bucket in the context; khrr_accel with tune == NULL uses the cache (or a
+ * static heuristic when the bucket was never tuned). */
 int khrr_tune_gravity(khrr_ctx* ctx, int n, khrr_tune* best, double* best_ms);
 /* Same for the batched form on the composition host_sys_off[0..n_sys]
- * (i_per_lane, waves_per_block; j_split follows the per-system rule
- * there).  The result is
- * what khrr_accel_batched uses when its tune argument is NULL. */
+ * (i_per_lane, waves_per_block, then j_tile; j_split and the superblock
+ * follow the per-system rules there).  The result is what
+ * khrr_accel_batched uses when its tune argument is NULL. */
 int khrr_tune_batched(khrr_ctx* ctx, int n_sys, const int* host_sys_off, khrr_tune* best,
                       double* best_ms);
 /* Static choice the library makes for n when nothing is tuned. */
 int khrr_tune_default(khrr_ctx* ctx, int n, khrr_tune* out);
+/* The shape khrr_accel uses for n bodies when its tune argument is NULL,
+ * if one was tuned or loaded for n's size bucket (ceil(log2 n));
+ * KHRR_ESTATE otherwise.  khrr_tune_batched_cached: the same for
+ * khrr_accel_batched. */
+int khrr_tune_cached(khrr_ctx* ctx, int n, khrr_tune* out);
+int khrr_tune_batched_cached(khrr_ctx* ctx, khrr_tune* out);
+/* The autotuner's cache on disk.  khrr_tune_save writes every tuned shape
+ * of the context (per size bucket, with its trial time, and the batched
+ * shape) as JSON, tagged with the device's architecture name and CU count.
+ * khrr_tune_load reads such a file into the context: KHRR_ESTATE when it
+ * was made on another kind of device, KHRR_EINVAL when it is malformed or
+ * holds an invalid shape (then nothing is loaded).  Host-side, no device
+ * work. */
+int khrr_tune_save(khrr_ctx* ctx, const char* path);
+int khrr_tune_load(khrr_ctx* ctx, const char* path);
 
 /* ---- integrator (a3) ------------------------------------------------- */
 /* DataBlock fields used by dev_fex (common.cuh:40-56) */

@@ -228,7 +228,7 @@ BatchLauncher pick_batched(int ipl, bool a, bool p, bool c) {
 // more superblocks of 1024, 2048 or 4096 bodies: the one-sided diagonal
 // items are at most 1/16 of the system's work).  Like the piece count, a
 // rule of the system's size.
-inline bool on_sym_path(int n) { return n >= 16384 && khrr::sym_eligible(n, 8); }
+inline bool on_sym_path(int n) { return n >= 16384 && khrr::sym_eligible(n, 8, 0); }
 
 // Device work list of one composition.
 struct WorkList {
@@ -292,7 +292,7 @@ int prepare_items(khrr_ctx* ctx, int n_sys, const int* host_begin, const int* ho
     for (int s = 0; s < n_sys; ++s) {
       flag_base[s] = (int)tiles.size();
       const int b = host_begin[s], n = host_count[s], e = b + n;
-      const int SB = on_sym_path(n) ? khrr::sym_superblock(n) : 0;
+      const int SB = on_sym_path(n) ? khrr::sym_superblock(n, 0) : 0;
       const int step = SB ? SB : j_tile;
       for (int jt = b; jt < e; jt += step) tiles.push_back(make_int2(jt, std::min(e, jt + step)));
       if (!SB) continue;
@@ -533,7 +533,7 @@ int round_ipl(const khrr_ctx* ctx, int n_sys, const int* count, int pref) {
 size_t batched_part_bytes(int n, bool compensated) {
   if (n <= 0) return 0;
   if (on_sym_path(n)) {
-    const int sb = sym_superblock(n);
+    const int sb = sym_superblock(n, 0);
     const size_t n_sb = (size_t)(n + sb - 1) / sb;
     return n_sb * 4 * (size_t)n * sizeof(float);
   }
@@ -730,6 +730,7 @@ int khrr_accel_batched(khrr_ctx* ctx, int n_sys, const int* sys_off, const int* 
     t.i_per_lane = khrr::round_ipl(ctx, n_sys, count.data(), 4);
   }
   t.j_split = 1;
+  t.superblock = 0;  // pair-symmetric systems follow the rule of their own n
   KHRR_TRY(khrr::validate_tune(ctx, t));
   (void)sys_off;  // the host offsets define the segments
   return batched_launch(ctx, n_sys, host_sys_off, pos, h, m, acc, phi, flags, t);
@@ -755,34 +756,44 @@ int khrr_tune_gravity(khrr_ctx* ctx, int n, khrr_tune* best, double* best_ms) {
   }
   double best_t = 1e300;
   khrr_tune best_c = khrr::default_tune(ctx, n, n);
-  const int ipls[] = {1, 2, 4, 8};
-  const int wpbs[] = {1, 2, 4};
-  const int splits[] = {1, 2, 4, 8, 16};
   const bool timing = ctx->timing;
   ctx->timing = false;
-  for (int ipl : ipls) {
-    for (int wpb : wpbs) {
-      for (int sp : splits) {
+  auto try_one_sided = [&](const khrr_tune& c) {
+    double ms = 0.0;
+    rc = time_candidate(ctx, [&] {
+      return khrr::accel_launch(ctx, n, pos, h, nj, pos, m, h, acc, nullptr, 0, c);
+    }, &ms);
+    if (!rc && ms < best_t) {
+      best_t = ms;
+      best_c = c;
+    }
+  };
+  // One-sided: i-bodies per lane x waves per block x j-ranges with 256-body
+  // tiles, then the j-tile of the fastest of them.
+  for (int ipl : {1, 2, 4, 8})
+    for (int wpb : {1, 2, 4})
+      for (int sp : {1, 2, 4, 8, 16}) {
         if (rc) break;
         if (sp > 1 && (long long)nj / sp < 1024) continue;
-        khrr_tune c{ipl, wpb, sp, 256, 0};
-        double ms = 0.0;
-        rc = time_candidate(ctx, [&] {
-          return khrr::accel_launch(ctx, n, pos, h, nj, pos, m, h, acc, nullptr, 0, c);
-        }, &ms);
-        if (!rc && ms < best_t) {
-          best_t = ms;
-          best_c = c;
-        }
+        try_one_sided(khrr_tune{ipl, wpb, sp, 256, 0, 0});
       }
-    }
+  const khrr_tune base = best_c;
+  for (int jt : {128, 512, 1024}) {
+    if (rc) break;
+    if (jt > nj) continue;
+    khrr_tune c = base;
+    c.j_tile = jt;
+    try_one_sided(c);
   }
-  // Pair-symmetric candidates run on the full system (i-set == j-set) and
-  // are compared per pair-interaction with the one-sided ones.
+  // Pair-symmetric candidates (i-bodies per lane x superblock size) run on
+  // the full system (i-set == j-set) and are compared per pair-interaction
+  // with the one-sided ones.
   double best_per_pair = best_t / ((double)n * nj);
-  for (int ipl : {4, 8}) {
-    if (rc || !khrr::sym_eligible(n, ipl)) break;
-    khrr_tune c{ipl, best_c.waves_per_block, best_c.j_split, 256, 1};
+  const int sym_shapes[][2] = {{4, 1024}, {4, 2048}, {8, 2048}, {4, 4096}, {8, 4096}};
+  for (const auto& sh : sym_shapes) {
+    if (rc) break;
+    if (!khrr::sym_eligible(n, sh[0], sh[1])) continue;
+    khrr_tune c{sh[0], best_c.waves_per_block, best_c.j_split, best_c.j_tile, 1, sh[1]};
     double ms = 0.0;
     rc = time_candidate(ctx, [&] {
       return khrr::accel_launch(ctx, n, pos, h, n, pos, m, h, acc, nullptr, 0, c);
@@ -801,6 +812,7 @@ int khrr_tune_gravity(khrr_ctx* ctx, int n, khrr_tune* best, double* best_ms) {
   if (acc) (void)hipFree(acc);
   if (rc) return rc;
   khrr::store_tune(ctx, n, best_c);
+  ctx->tuned_ms[khrr::tune_bucket(n)] = best_t;
   *best = best_c;
   if (best_ms) *best_ms = best_t;
   return 0;
@@ -831,22 +843,33 @@ int khrr_tune_batched(khrr_ctx* ctx, int n_sys, const int* host_sys_off, khrr_tu
                  hipSuccess)
     rc = khrr::fail(ctx, KHRR_EDEVICE, "offset upload failed");
   double best_t = 1e300;
-  khrr_tune best_c{4, 4, 1, 256};
+  khrr_tune best_c{4, 4, 1, 256, 0, 0};
   const bool timing = ctx->timing;
   ctx->timing = false;
-  for (int ipl : {2, 4, 8}) {
+  auto try_shape = [&](const khrr_tune& c) {
+    double ms = 0.0;
+    rc = time_candidate(ctx, [&] {
+      return batched_launch(ctx, n_sys, host_sys_off, pos, h, m, acc, nullptr, 0, c);
+    }, &ms);
+    if (!rc && ms < best_t) {
+      best_t = ms;
+      best_c = c;
+    }
+  };
+  // i-bodies per lane x waves per block with 256-body j-tiles, then the
+  // j-tile of the fastest (it sets the one-sided items' partial-sum tiles
+  // and so their results; the pair-symmetric systems follow their rule)
+  for (int ipl : {2, 4, 8})
     for (int wpb : {1, 2, 4}) {
       if (rc) break;
-      khrr_tune c{ipl, wpb, 1, 256};
-      double ms = 0.0;
-      rc = time_candidate(ctx, [&] {
-        return batched_launch(ctx, n_sys, host_sys_off, pos, h, m, acc, nullptr, 0, c);
-      }, &ms);
-      if (!rc && ms < best_t) {
-        best_t = ms;
-        best_c = c;
-      }
+      try_shape(khrr_tune{ipl, wpb, 1, 256, 0, 0});
     }
+  const khrr_tune base = best_c;
+  for (int jt : {128, 512, 1024}) {
+    if (rc) break;
+    khrr_tune c = base;
+    c.j_tile = jt;
+    try_shape(c);
   }
   ctx->timing = timing;
   (void)hipStreamSynchronize(ctx->stream);

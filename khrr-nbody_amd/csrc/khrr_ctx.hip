@@ -136,8 +136,10 @@ int khrr_ctx_create(int device, khrr_ctx** out) {
   khrr_ctx* ctx = new khrr_ctx();
   ctx->device = device;
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
-    ctx->cu_count = prop.multiProcessorCount;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    if (prop.multiProcessorCount > 0) ctx->cu_count = prop.multiProcessorCount;
+    ctx->arch = prop.gcnArchName;
+  }
   if (hipMalloc(&ctx->dev_words, 256) != hipSuccess) {
     delete ctx;
     return khrr::fail(nullptr, KHRR_ENOMEM, "context word allocation failed");

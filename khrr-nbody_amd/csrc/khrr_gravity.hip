@@ -192,11 +192,6 @@ GravLauncher pick(int ipl, bool do_acc, bool do_phi, bool comp) {
   }
 }
 
-int log2_bucket(int n) {
-  int b = 0;
-  while ((1ll << b) < (long long)n && b < 39) ++b;
-  return b;
-}
 
 }  // namespace
 
@@ -215,6 +210,9 @@ int validate_tune(khrr_ctx* ctx, const khrr_tune& t) {
                 t.j_tile);
   if (t.symmetric != 0 && t.symmetric != 1)
     return fail(ctx, KHRR_EINVAL, "tune.symmetric must be 0 or 1 (got %d)", t.symmetric);
+  if (t.superblock != 0 && t.superblock != 1024 && t.superblock != 2048 && t.superblock != 4096)
+    return fail(ctx, KHRR_EINVAL, "tune.superblock must be 0, 1024, 2048 or 4096 (got %d)",
+                t.superblock);
   return 0;
 }
 
@@ -230,6 +228,7 @@ khrr_tune default_tune(const khrr_ctx* ctx, int n_i, int n_j) {
   t.waves_per_block = 4;
   t.j_tile = 256;
   t.symmetric = 0;
+  t.superblock = 0;
   const long long groups = (n_i + 64ll * t.i_per_lane - 1) / (64ll * t.i_per_lane);
   const long long target = 32ll * ctx->cu_count;
   long long split = groups > 0 ? (target + groups - 1) / groups : 1;
@@ -241,13 +240,19 @@ khrr_tune default_tune(const khrr_ctx* ctx, int n_i, int n_j) {
   return t;
 }
 
+int tune_bucket(int n) {
+  int b = 0;
+  while ((1ll << b) < (long long)n && b < kTuneBuckets - 1) ++b;
+  return b;
+}
+
 const khrr_tune* cached_tune(const khrr_ctx* ctx, int n) {
-  const int b = log2_bucket(n);
+  const int b = tune_bucket(n);
   return ctx->has_tuned[b] ? &ctx->tuned[b] : nullptr;
 }
 
 void store_tune(khrr_ctx* ctx, int n, const khrr_tune& t) {
-  const int b = log2_bucket(n);
+  const int b = tune_bucket(n);
   ctx->tuned[b] = t;
   ctx->has_tuned[b] = true;
 }
@@ -274,8 +279,9 @@ int accel_launch(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, i
     return 0;
   }
   if (tune.symmetric && n_i == n_j && pos_i == pos_j && h_i == h_j &&
-      sym_eligible(n_i, tune.i_per_lane)) {
-    const int rc = sym_launch(ctx, n_i, pos_i, h_i, m_j, acc, phi, flags, tune.i_per_lane);
+      sym_eligible(n_i, tune.i_per_lane, tune.superblock)) {
+    const int rc = sym_launch(ctx, n_i, pos_i, h_i, m_j, acc, phi, flags, tune.i_per_lane,
+                              tune.superblock);
     if (rc != KHRR_ENOMEM) return rc;
     // no room for the partial slots: the one-sided kernel needs none
   }
@@ -350,7 +356,7 @@ int khrr_accel(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int
   } else {
     t = khrr::default_tune(ctx, n_i, n_j);
     const bool same_set = n_i == n_j && pos_i == pos_j && h_i == h_j;
-    if (same_set && khrr::sym_eligible(n_i, 8) && khrr::sym_worthwhile(ctx, n_i)) {
+    if (same_set && khrr::sym_eligible(n_i, 8, 0) && khrr::sym_worthwhile(ctx, n_i)) {
       t.i_per_lane = 8;
       t.symmetric = 1;
     }

@@ -63,6 +63,7 @@ struct khrr_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   int cu_count = 256;
+  std::string arch;  // gcnArchName of the device (keys the tuning cache file)
   // Scratch arena: one device allocation grown on demand; callers carve
   // stream-ordered sub-buffers out of it for the duration of one call.
   void* scratch = nullptr;
@@ -77,9 +78,10 @@ struct khrr_ctx {
   std::vector<hipEvent_t> event_pool;
   long launches[KHRR_K_COUNT] = {0};
   double total_ms[KHRR_K_COUNT] = {0};
-  // Autotuner cache, indexed by ceil(log2(n)).
+  // Autotuner cache, indexed by ceil(log2(n)) (kTuneBuckets entries).
   khrr_tune tuned[40] = {};
   bool has_tuned[40] = {};
+  double tuned_ms[40] = {};
   khrr_tune batch_tuned = {};
   bool has_batch_tuned = false;
   // Batched form: the work list (system, i-group) of the last composition,
@@ -166,6 +168,9 @@ inline unsigned ceil_div(long long a, long long b) { return unsigned((a + b - 1)
 int validate_tune(khrr_ctx* ctx, const khrr_tune& t);
 khrr_tune default_tune(const khrr_ctx* ctx, int n_i, int n_j);
 const khrr_tune* cached_tune(const khrr_ctx* ctx, int n);
+constexpr int kTuneBuckets = 40;
+// Size bucket of the tuning cache: ceil(log2(n)), at most kTuneBuckets - 1.
+int tune_bucket(int n);
 void store_tune(khrr_ctx* ctx, int n, const khrr_tune& t);
 int accel_launch(khrr_ctx* ctx, int n_i, const float* pos_i, const float* h_i, int n_j,
                  const float* pos_j, const float* m_j, const float* h_j, float* acc, float* phi,
@@ -227,16 +232,18 @@ int noalloc_init(khrr_ctx* ctx);
 // size[1], y stride size[2]) never past the last cell.
 int check_grid(khrr_ctx* ctx, const khrr_grid* grid);
 // Pair-symmetric form (khrr_sym.hip) for one system (i-set == j-set).
-// Bodies per superblock for a system of n bodies: 4096 from 524288 bodies
-// on, else 2048 (a rule of n alone, shared by the plain and batched forms).
-int sym_superblock(int n);
-// i-bodies per lane the pair-symmetric kernel runs for n bodies when asked
-// for ipl (1024-body superblocks: always 4).
-int sym_ipl(int n, int ipl);
-bool sym_eligible(int n, int ipl);
+// Bodies per superblock: `requested` when it is 1024, 2048 or 4096, else
+// the rule of n alone -- 4096 from 524288 bodies on, 2048 from 32768, 1024
+// below (shared by the plain and batched forms; the batched form always
+// follows the rule).
+int sym_superblock(int n, int requested);
+// i-bodies per lane the pair-symmetric kernel runs for superblocks of sb
+// bodies when asked for ipl (1024-body superblocks: always 4).
+int sym_ipl(int sb, int ipl);
+bool sym_eligible(int n, int ipl, int requested);
 bool sym_worthwhile(const khrr_ctx* ctx, int n);
 int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const float* m, float* acc,
-               float* phi, unsigned flags, int ipl);
+               float* phi, unsigned flags, int ipl, int requested);
 // Batched pair-symmetric launches (i_per_lane 8) over a device item table
 // holding n_big items of 4096-body superblocks then n_small of 2048, then
 // the per-system slot reduction; max_n bounds every system.

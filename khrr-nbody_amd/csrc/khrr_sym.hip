@@ -393,13 +393,18 @@ constexpr size_t kSymPartBudget = size_t(40) << 30;
 // partly filled wave of workgroups; from 524288 bodies on the 4096 form has
 // items enough and half the partial-slot memory.  Below 32768 bodies,
 // 1024-body superblocks (4 i per lane, 4 waves) keep 16 or more of them.
-int sym_superblock(int n) { return n >= 524288 ? 4096 : n >= 32768 ? 2048 : 1024; }
+// A tuned launch shape may ask for any of the three sizes instead.
+int sym_superblock(int n, int requested) {
+  if (requested == 1024 || requested == 2048 || requested == 4096) return requested;
+  return n >= 524288 ? 4096 : n >= 32768 ? 2048 : 1024;
+}
 
-int sym_ipl(int n, int ipl) { return sym_superblock(n) == 1024 ? 4 : ipl; }
+int sym_ipl(int sb, int ipl) { return sb == 1024 ? 4 : ipl; }
 
-bool sym_eligible(int n, int ipl) {
+bool sym_eligible(int n, int ipl, int requested) {
   if (ipl != 4 && ipl != 8) return false;
-  const int sb = sym_superblock(n);
+  if (requested != 0 && requested != 1024 && requested != 2048 && requested != 4096) return false;
+  const int sb = sym_superblock(n, requested);
   const long long n_sb = (n + (long long)sb - 1) / sb;
   if (n_sb < 2) return false;
   return (size_t)n_sb * 4 * (size_t)n * sizeof(float) <= kSymPartBudget;
@@ -407,19 +412,19 @@ bool sym_eligible(int n, int ipl) {
 
 // Enough (A, B) items to fill the device (about two workgroups per CU).
 bool sym_worthwhile(const khrr_ctx* ctx, int n) {
-  const int sb = sym_superblock(n);
+  const int sb = sym_superblock(n, 0);
   const long long n_sb = (n + (long long)sb - 1) / sb;
   return n_sb * (n_sb + 1) / 2 >= 2ll * ctx->cu_count;
 }
 
 int sym_launch(khrr_ctx* ctx, int n, const float* pos, const float* h, const float* m, float* acc,
-               float* phi, unsigned flags, int ipl) {
+               float* phi, unsigned flags, int ipl, int requested) {
   const bool do_acc = acc != nullptr, do_phi = phi != nullptr;
   const bool comp = (flags & KHRR_COMPENSATED) != 0;
   const int accumulate = (flags & KHRR_ACCUMULATE) ? 1 : 0;
-  const int sb = sym_superblock(n);
+  const int sb = sym_superblock(n, requested);
   const int n_sb = (n + sb - 1) / sb;
-  ipl = sym_ipl(n, ipl);
+  ipl = sym_ipl(sb, ipl);
   const size_t part_elems = (size_t)n_sb * 4 * n;
   void* base = nullptr;
   KHRR_TRY(scratch(ctx, Carver::need(n_sb, sizeof(int)) + Carver::need(part_elems, sizeof(float)),

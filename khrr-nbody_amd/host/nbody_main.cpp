@@ -63,13 +63,15 @@ struct Options {
   bool wave = false;
   int setup_iters = 20000;  // _iterations_to_setup, nbody.cu:1143
   bool debug_dumps = false;
+  std::string tune_cache;  // autotuner cache (JSON) to load, or empty
 };
 
 void usage() {
   std::fprintf(stderr,
                "usage: khrr_nbody [--data DIR] [--compat-accumulate] [--compensated]\n"
                "                  [--clear-cell-phi] [--max-steps N] [--grid NX DX] [--json]\n"
-               "                  [--solver nbody|wave] [--setup-iters N] [--debug-dumps]\n");
+               "                  [--solver nbody|wave] [--setup-iters N] [--debug-dumps]\n"
+               "                  [--tune-cache FILE]\n");
 }
 
 bool parse(int argc, char** argv, Options& o) {
@@ -99,6 +101,8 @@ bool parse(int argc, char** argv, Options& o) {
       o.wave = v == "wave";
     } else if (a == "--debug-dumps") {
       o.debug_dumps = true;
+    } else if (a == "--tune-cache" && i + 1 < argc) {
+      o.tune_cache = argv[++i];
     } else if (a == "--setup-iters" && i + 1 < argc) {
       o.setup_iters = std::atoi(argv[++i]);
       if (o.setup_iters < 0) return false;
@@ -419,6 +423,11 @@ int run(const Options& opt) {
     khrr_ctx* c;
     ~CtxGuard() { khrr_ctx_destroy(c); }
   } guard{ctx};
+  // launch shapes tuned earlier on this kind of device (khrr_tune_save)
+  if (!opt.tune_cache.empty() && khrr_tune_load(ctx, opt.tune_cache.c_str()) != 0) {
+    std::fprintf(stderr, "Error: %s\n", khrr_last_error(ctx));
+    return 1;
+  }
 
   // host -> device (FP64 state; FP32 gravity inputs with h = eps^2/2)
   std::vector<float> pos32(3 * N), m32(N), h32(N);

@@ -50,7 +50,8 @@ class KhrrError(RuntimeError):
 
 class Tune(ctypes.Structure):
     _fields_ = [("i_per_lane", ctypes.c_int), ("waves_per_block", ctypes.c_int),
-                ("j_split", ctypes.c_int), ("j_tile", ctypes.c_int), ("symmetric", ctypes.c_int)]
+                ("j_split", ctypes.c_int), ("j_tile", ctypes.c_int), ("symmetric", ctypes.c_int),
+                ("superblock", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -109,6 +110,10 @@ _SIGS = {
     "khrr_tune_gravity": (_I, [_P, _I, ctypes.POINTER(Tune), ctypes.POINTER(_D)]),
     "khrr_tune_default": (_I, [_P, _I, ctypes.POINTER(Tune)]),
     "khrr_tune_batched": (_I, [_P, _I, _P, ctypes.POINTER(Tune), ctypes.POINTER(_D)]),
+    "khrr_tune_cached": (_I, [_P, _I, ctypes.POINTER(Tune)]),
+    "khrr_tune_batched_cached": (_I, [_P, ctypes.POINTER(Tune)]),
+    "khrr_tune_save": (_I, [_P, ctypes.c_char_p]),
+    "khrr_tune_load": (_I, [_P, ctypes.c_char_p]),
     "khrr_extpot_init": (_I, [_D, _D, _D, _D, _D, _D, ctypes.POINTER(ExtPot)]),
     "khrr_extpot_force": (_I, [ctypes.POINTER(ExtPot), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "khrr_integrate": (_I, [_P, _I, _I, _D, _D, ctypes.POINTER(ExtPot), _P, _P, _P, _P, _P,
@@ -317,6 +322,31 @@ class Context:
         t = Tune()
         self._check(self._lib.khrr_tune_default(self._h, n, ctypes.byref(t)))
         return t
+
+    def tune_cached(self, n: int):
+        """The tuned shape khrr_accel uses for n bodies, or None."""
+        t = Tune()
+        rc = self._lib.khrr_tune_cached(self._h, n, ctypes.byref(t))
+        if rc == -4:  # KHRR_ESTATE: nothing tuned for this size bucket
+            return None
+        self._check(rc)
+        return t
+
+    def tune_batched_cached(self):
+        t = Tune()
+        rc = self._lib.khrr_tune_batched_cached(self._h, ctypes.byref(t))
+        if rc == -4:
+            return None
+        self._check(rc)
+        return t
+
+    def tune_save(self, path: str):
+        """Writes the context's tuned shapes to a JSON cache file."""
+        self._check(self._lib.khrr_tune_save(self._h, os.fsencode(path)))
+
+    def tune_load(self, path: str):
+        """Loads a JSON cache file made on the same kind of device."""
+        self._check(self._lib.khrr_tune_load(self._h, os.fsencode(path)))
 
     # -- integrator ----------------------------------------------------
     def integrate(self, istep, dt, t, acc, pos_t, vel_t, pos, vel, acc0=None, pos32=None,

@@ -28,7 +28,7 @@ def test_binding_covers_header():
 
 
 def test_abi_version():
-    assert khrr.lib().khrr_abi_version() == 1
+    assert khrr.lib().khrr_abi_version() == 2  # the superblock field of khrr_tune
 
 
 def test_extpot_constants_match_reference_formulas():

@@ -40,14 +40,14 @@ def _run_batched(ctx, off, pos, m, h, tune=None, phi=False, compensated=False):
     return acc.cpu().numpy(), (ph.cpu().numpy() if phi else None)
 
 
-def _plain_tune(n):
+def _plain_tune(n, j_tile=256):
     import khrr
     if n >= 16384:
         return khrr.Tune(8, 4, 1, 256, 1)
-    return khrr.Tune(4, 4, max(1, min(32, -(-n // 1024))), 256, 0)
+    return khrr.Tune(4, 4, max(1, min(32, -(-n // 1024))), j_tile, 0)
 
 
-def _run_plain(ctx, pos, m, h, phi=False, compensated=False):
+def _run_plain(ctx, pos, m, h, phi=False, compensated=False, j_tile=256):
     import torch
     n = pos.shape[1]
     acc = torch.zeros((3, n), device="cuda:0")
@@ -55,7 +55,7 @@ def _run_plain(ctx, pos, m, h, phi=False, compensated=False):
     p = to_dev(pos, torch.float32)
     hh = to_dev(h, torch.float32)
     ctx.accel(p, hh, p, to_dev(m, torch.float32), hh, acc=acc, phi=ph,
-              tune=_plain_tune(n), compensated=compensated)
+              tune=_plain_tune(n, j_tile), compensated=compensated)
     torch.cuda.synchronize()
     return acc.cpu().numpy(), (ph.cpu().numpy() if phi else None)
 
@@ -151,18 +151,27 @@ def test_bad_offsets_rejected(ctx):
                           [0, 3, 2, 4], pos, h, h, acc=torch.zeros((3, 4), device="cuda:0"))
 
 
-def test_autotuners_return_valid_shapes(ctx):
-    t, ms = ctx.tune_gravity(20000)
-    assert t.i_per_lane in (1, 2, 4, 8) and 1 <= t.waves_per_block <= 4 and ms > 0
-    off, pos, m, h = _population([1024, 2048, 4096], seed0=600)
-    tb, msb = ctx.tune_batched(off.tolist())
-    assert tb.i_per_lane in (2, 4, 8) and tb.j_split == 1 and msb > 0
-    # the tuned shape is what a NULL tune now uses; results stay exact
-    acc, _ = _run_batched(ctx, off, pos, m, h)
-    for s in range(3):
-        a, b = off[s], off[s + 1]
-        pa, _ = _run_plain(ctx, pos[:, a:b], m[a:b], h[a:b])
-        np.testing.assert_array_equal(acc[:, a:b], pa)
+def test_autotuners_return_valid_shapes():
+    # a context of its own: the tuned shapes it stores (a tuned j-tile
+    # changes the one-sided sums) stay out of the other tests' context
+    import khrr
+    ctx = khrr.Context(0)
+    try:
+        t, ms = ctx.tune_gravity(20000)
+        assert t.i_per_lane in (1, 2, 4, 8) and 1 <= t.waves_per_block <= 4 and ms > 0
+        off, pos, m, h = _population([1024, 2048, 4096], seed0=600)
+        tb, msb = ctx.tune_batched(off.tolist())
+        assert tb.i_per_lane in (2, 4, 8) and tb.j_split == 1 and msb > 0
+        assert tb.j_tile in (128, 256, 512, 1024)
+        # the tuned shape is what a NULL tune now uses; each system equals
+        # its plain launch with the same j-tile bit for bit
+        acc, _ = _run_batched(ctx, off, pos, m, h)
+        for s in range(3):
+            a, b = off[s], off[s + 1]
+            pa, _ = _run_plain(ctx, pos[:, a:b], m[a:b], h[a:b], j_tile=tb.j_tile)
+            np.testing.assert_array_equal(acc[:, a:b], pa)
+    finally:
+        ctx.close()
 
 
 def test_graph_replays_equal_eager(ctx):

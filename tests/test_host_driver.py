@@ -306,3 +306,30 @@ def test_restart_continues_bit_for_bit(tmp_path):
     raw = open(os.path.join(b, "bin", "S_   20.bin"), "rb").read()
     assert struct.unpack("<i", raw[:4])[0] == 1024
     assert abs(struct.unpack("<d", raw[4:12])[0] - 0.04) < 1e-12
+
+
+@pytest.mark.gpu
+def test_tune_cache_flag(tmp_path):
+    # --tune-cache loads a khrr_tune_save file into the driver's context; a
+    # file made on another kind of device is refused before any step
+    import json
+
+    import khrr
+    c = khrr.Context(0)
+    try:
+        c.tune_gravity(3072)
+        good = str(tmp_path / "tune.json")
+        c.tune_save(good)
+    finally:
+        c.close()
+    base = _layout(str(tmp_path), [dict(n_s=1024, n_dm=2048, seed=13)], 0.004, 0.002)
+    r = subprocess.run([BIN, "--data", base, "--grid", "16", "0.25", "--tune-cache", good],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    doc = json.load(open(good))
+    doc["arch"] = "gfx000"
+    bad = str(tmp_path / "other.json")
+    json.dump(doc, open(bad, "w"))
+    r = subprocess.run([BIN, "--data", base, "--grid", "16", "0.25", "--tune-cache", bad],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 1 and "tuned on gfx000" in r.stderr, r.stderr[-2000:]
