@@ -90,6 +90,17 @@ def parse():
                     help="time the candidate shapes again instead of using the cache")
     ap.add_argument("--tune-save", default=None,
                     help="write the context's tuned shapes to this JSON file at the end")
+    ap.add_argument("--stream-requests", type=int, default=5000,
+                    help="length of the serving stream (--stream-range runs a segment of it)")
+    ap.add_argument("--stream-range", default=None,
+                    help="first,last: serve requests [first, last) of the stream in real time "
+                         "and write their records to --stream-out (no other leg runs)")
+    ap.add_argument("--stream-out", default="gpurun_out/stream.json")
+    ap.add_argument("--stream-fifo", type=int, default=0,
+                    help="also run the segment's first K requests one at a time (plain FIFO)")
+    ap.add_argument("--stream-merge", default=None,
+                    help="comma list of --stream-out files of consecutive segments: print "
+                         "their combined figures (no GPU)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 512 MiB device copy first (counter calibration under rocprof)")
     return ap.parse_args()
@@ -468,27 +479,49 @@ def bench_batched_run(ctx, steps, budget_gb=32.0):
                     "mass 1 per system, so dP and dL are absolute"}
 
 
-def serving_requests(n_req, seed=7, tmax_lo=0.2, tmax_hi=2.0, dt=0.002):
-    """Mixed request stream: 90 % Plummer spheres (N log-uniform on
-    [1024, 131072], multiple of 1024, eps 0.004 (N/16384)^-1/3), 10 % disk +
-    halo galaxies (65536 + 131072); tmax uniform on [tmax_lo, tmax_hi]."""
+def serving_specs(n_req, seed=7, tmax_lo=0.2, tmax_hi=2.0, dt=0.002):
+    """The mixed request stream's shapes: 90 % Plummer spheres (N
+    log-uniform on [1024, 131072], multiple of 1024), 10 % disk + halo
+    galaxies (65536 + 131072); tmax uniform on [tmax_lo, tmax_hi].  Request
+    r's bodies come from seed 200000 + r (request_bodies)."""
     import numpy as np
-    import ics
     rng = np.random.default_rng(seed)
-    reqs = []
+    specs = []
     for r in range(n_req):
         if rng.uniform() < 0.1:
-            ps, vs, pd, vd = ics.galaxy(65536, 131072, seed=200000 + r)
-            pos, vel, mass, _ = ics.system_arrays(ps, vs, pd, vd, 1.0, 6.699356, 0.004, 0.004)
-            eps = np.full(pos.shape[1], 0.004)
+            kind, n = "galaxy", 65536 + 131072
         else:
+            kind = "plummer"
             n = int(np.clip(round(np.exp(rng.uniform(np.log(1024), np.log(131072))) / 1024) * 1024,
                             1024, 131072))
-            pos, vel = ics.plummer(n, 1.0, 0.3333, 1.0, seed=200000 + r)
-            mass = np.full(n, 1.0 / n)
-            eps = np.full(n, 0.004 * (n / 16384.0) ** (-1.0 / 3.0))
         steps = int(round(rng.uniform(tmax_lo, tmax_hi) / dt))
-        reqs.append(dict(pos=pos, vel=vel, mass=mass, eps=eps, steps=steps, n=pos.shape[1]))
+        specs.append(dict(r=r, kind=kind, n=n, steps=steps))
+    return specs
+
+
+def request_bodies(spec):
+    """Bodies of one request: (pos, vel, mass, eps) host arrays.  Plummer
+    spheres have mass 1, scale 0.3333, truncation 1.0 and eps 0.004 (N /
+    16384)^-1/3; galaxies are the headline galaxy's model (eps 0.004)."""
+    import numpy as np
+    import ics
+    seed = 200000 + spec["r"]
+    if spec["kind"] == "galaxy":
+        ps, vs, pd, vd = ics.galaxy(65536, 131072, seed=seed)
+        pos, vel, mass, _ = ics.system_arrays(ps, vs, pd, vd, 1.0, 6.699356, 0.004, 0.004)
+        return pos, vel, mass, np.full(pos.shape[1], 0.004)
+    n = spec["n"]
+    pos, vel = ics.plummer(n, 1.0, 0.3333, 1.0, seed=seed)
+    return pos, vel, np.full(n, 1.0 / n), np.full(n, 0.004 * (n / 16384.0) ** (-1.0 / 3.0))
+
+
+def serving_requests(n_req, seed=7, tmax_lo=0.2, tmax_hi=2.0, dt=0.002):
+    """The first n_req requests of the stream with their bodies."""
+    reqs = []
+    for spec in serving_specs(n_req, seed, tmax_lo, tmax_hi, dt):
+        pos, vel, mass, eps = request_bodies(spec)
+        reqs.append(dict(pos=pos, vel=vel, mass=mass, eps=eps, steps=spec["steps"],
+                         n=pos.shape[1]))
     return reqs
 
 
@@ -499,86 +532,80 @@ def poisson_arrivals(n_req, lam, seed=8):
     return arrivals - arrivals[0]
 
 
-def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4e10,
-                  lam_saturated=50.0, tmax=(0.2, 2.0)):
-    """Continuous batching on a Poisson arrival stream vs the same requests
-    run one at a time (FIFO) with plain per-system launches; then the same
-    requests arriving faster than either can serve them (lam_saturated)."""
+def serve_realtime(ctx, reqs, arrivals, budget_gb, max_active, round_pairs, grid, cells,
+                   label=""):
+    """Real-time arrivals into one server (request k submitted once
+    arrivals[k] seconds have passed); returns the latencies and figures.
+    busy: wall time while the server holds work (from the step that finds
+    it non-empty to the step that drains it; a draining step finishes a
+    request and so waits for the device)."""
+    import numpy as np
+    import khrr
+    n_req = len(reqs)
+    pairs = sum(r["n"] ** 2 * (r["steps"] + 3) for r in reqs)  # init acc+phi, steps, final phi
+    srv = khrr.Server(ctx, int(budget_gb * 1e9), max_active, round_pairs=round_pairs)
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    ids, lat = {}, {}
+    busy = 0.0
+    busy_since = None
+    t0 = time.perf_counter()
+    last_log = t0
+    nxt = 0
+    while len(lat) < n_req:
+        now = time.perf_counter() - t0
+        while nxt < n_req and arrivals[nxt] <= now:
+            r = reqs[nxt]
+            ids[srv.submit(r["pos"], r["vel"], r["mass"], r["eps"], 0.002, r["steps"], grid)] = nxt
+            nxt += 1
+        st = srv.stats()
+        if st["active"] == 0 and st["pending"] == 0:
+            time.sleep(max(0.0, min(0.01, arrivals[nxt] - now)) if nxt < n_req else 0.0)
+            continue
+        if busy_since is None:
+            busy_since = time.perf_counter()
+        finished = srv.step()
+        for fid in finished:
+            k = ids[fid]
+            lat[k] = time.perf_counter() - t0 - arrivals[k]
+            srv.result(fid, cells)
+        if n_req >= 50 and time.perf_counter() - last_log > 30.0:
+            last_log = time.perf_counter()
+            log(f"{label}served {len(lat)}/{n_req}, submitted {nxt}, active {st['active']}, "
+                f"pending {st['pending']}, {last_log - t0:.0f} s")
+        st = srv.stats()
+        if st["active"] == 0 and st["pending"] == 0:
+            busy += time.perf_counter() - busy_since
+            busy_since = None
+    makespan = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    _, grav_ms = ctx.kernel_time(khrr.K_GRAVITY)
+    _, int_ms = ctx.kernel_time(khrr.K_INTEGRATE)
+    ctx.set_timing(False)
+    lat_s = np.array([lat[k] for k in range(n_req)])
+    return lat_s, {"round_pairs": round_pairs, "makespan_s": makespan, "busy_s": busy,
+                   "rounds": st["rounds"], "gravity_kernel_s": grav_ms * 1e-3,
+                   "integrator_kernel_s": int_ms * 1e-3,
+                   "pair_interactions": st["pair_interactions"],
+                   "Gpairs_per_s": pairs / busy / 1e9,
+                   "p50_latency_s": float(np.percentile(lat_s, 50)),
+                   "p99_latency_s": float(np.percentile(lat_s, 99))}
+
+
+def plain_service_times(ctx, reqs, grid, cells, label=""):
+    """Each request alone with plain per-system launches (one khrr_accel per
+    force evaluation, the step sequence of the drop-in), t = 0 and final
+    diagnostics and the mass grid: its service time in seconds."""
     import numpy as np
     import torch
-    import khrr
-    reqs = serving_requests(n_req, tmax_lo=tmax[0], tmax_hi=tmax[1])
-    arrivals = poisson_arrivals(n_req, lam)
-    grid = khrr.cube_grid(128, -2.0, 2.0)
-    cells = 128 ** 3
-    pairs = sum(r["n"] ** 2 * (r["steps"] + 3) for r in reqs)  # init acc+phi, steps, final phi
-
-    def served(round_pairs, arrivals=arrivals):
-        """Real-time arrivals into one server; returns latencies and figures.
-        busy: wall time while the server holds work (from the step that finds
-        it non-empty to the step that drains it; a draining step finishes a
-        request and so waits for the device)."""
-        srv = khrr.Server(ctx, int(budget_gb * 1e9), max_active, round_pairs=round_pairs)
-        ctx.reset_timing()
-        ctx.set_timing(True)
-        ids, lat = {}, {}
-        busy = 0.0
-        busy_since = None
-        t0 = time.perf_counter()
-        nxt = 0
-        while len(lat) < n_req:
-            now = time.perf_counter() - t0
-            while nxt < n_req and arrivals[nxt] <= now:
-                r = reqs[nxt]
-                ids[srv.submit(r["pos"], r["vel"], r["mass"], r["eps"], 0.002, r["steps"],
-                               grid)] = nxt
-                nxt += 1
-            st = srv.stats()
-            if st["active"] == 0 and st["pending"] == 0:
-                time.sleep(max(0.0, min(0.01, arrivals[nxt] - now)) if nxt < n_req else 0.0)
-                continue
-            if busy_since is None:
-                busy_since = time.perf_counter()
-            finished = srv.step()
-            for fid in finished:
-                k = ids[fid]
-                lat[k] = time.perf_counter() - t0 - arrivals[k]
-                srv.result(fid, cells)
-                if n_req >= 50 and len(lat) % 25 == 0:
-                    log(f"served {len(lat)}/{n_req}")
-            st = srv.stats()
-            if st["active"] == 0 and st["pending"] == 0:
-                busy += time.perf_counter() - busy_since
-                busy_since = None
-        makespan = time.perf_counter() - t0
-        st = srv.stats()
-        srv.close()
-        _, grav_ms = ctx.kernel_time(khrr.K_GRAVITY)
-        _, int_ms = ctx.kernel_time(khrr.K_INTEGRATE)
-        ctx.set_timing(False)
-        lat_s = np.array([lat[k] for k in range(n_req)])
-        return lat_s, {"round_pairs": round_pairs, "makespan_s": makespan, "busy_s": busy,
-                       "rounds": st["rounds"], "gravity_kernel_s": grav_ms * 1e-3,
-                       "integrator_kernel_s": int_ms * 1e-3,
-                       "pair_interactions": st["pair_interactions"],
-                       "Gpairs_per_s": pairs / busy / 1e9,
-                       "p50_latency_s": float(np.percentile(lat_s, 50)),
-                       "p99_latency_s": float(np.percentile(lat_s, 99))}
-
-    policies = {}
-    for name, rp in (("lockstep", 0.0), ("least_work_first", round_pairs)):
-        lat_s, fig = served(rp)
-        policies[name] = fig
-        log(f"served ({name}) {n_req} requests in {fig['makespan_s']:.2f} s, "
-            f"busy {fig['busy_s']:.2f} s, p50 {fig['p50_latency_s']:.3f} s, "
-            f"p99 {fig['p99_latency_s']:.3f} s")
-
-    # plain: each request alone, one khrr_accel per force evaluation
     dev = torch.device("cuda:0")
     service = []
+    last_log = time.perf_counter()
     for ri, r in enumerate(reqs):
-        if n_req >= 50 and ri % 25 == 0:
-            log(f"plain {ri}/{n_req}")
+        if len(reqs) >= 50 and time.perf_counter() - last_log > 30.0:
+            last_log = time.perf_counter()
+            log(f"{label}plain {ri}/{len(reqs)}")
         torch.cuda.synchronize()
         ts = time.perf_counter()
         sim = Sim(ctx, r["pos"], r["vel"], r["mass"], 0.5 * r["eps"] ** 2, 0.002)
@@ -594,15 +621,39 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4
         torch.cuda.synchronize()
         service.append(time.perf_counter() - ts)
         del sim
-    service = np.array(service)
+    return np.array(service)
 
-    def fifo_latency(arr):
-        done = np.zeros(n_req)
-        for k in range(n_req):  # FIFO queue with the same arrivals
-            done[k] = max(arr[k], done[k - 1] if k else 0.0) + service[k]
-        return done - arr
 
-    plain_lat = fifo_latency(arrivals)
+def fifo_latency(arrivals, service):
+    """Latencies of a FIFO queue serving one request at a time."""
+    import numpy as np
+    done = np.zeros(len(service))
+    for k in range(len(service)):
+        done[k] = max(arrivals[k], done[k - 1] if k else 0.0) + service[k]
+    return done - np.asarray(arrivals[:len(service)])
+
+
+def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4e10,
+                  lam_saturated=50.0, tmax=(0.2, 2.0)):
+    """Continuous batching on a Poisson arrival stream vs the same requests
+    run one at a time (FIFO) with plain per-system launches; then the same
+    requests arriving faster than either can serve them (lam_saturated)."""
+    import numpy as np
+    import khrr
+    reqs = serving_requests(n_req, tmax_lo=tmax[0], tmax_hi=tmax[1])
+    arrivals = poisson_arrivals(n_req, lam)
+    grid = khrr.cube_grid(128, -2.0, 2.0)
+    cells = 128 ** 3
+    pairs = sum(r["n"] ** 2 * (r["steps"] + 3) for r in reqs)  # init acc+phi, steps, final phi
+    policies = {}
+    for name, rp in (("lockstep", 0.0), ("least_work_first", round_pairs)):
+        lat_s, fig = serve_realtime(ctx, reqs, arrivals, budget_gb, max_active, rp, grid, cells)
+        policies[name] = fig
+        log(f"served ({name}) {n_req} requests in {fig['makespan_s']:.2f} s, "
+            f"busy {fig['busy_s']:.2f} s, p50 {fig['p50_latency_s']:.3f} s, "
+            f"p99 {fig['p99_latency_s']:.3f} s")
+    service = plain_service_times(ctx, reqs, grid, cells)
+    plain_lat = fifo_latency(arrivals, service)
     log(f"plain FIFO: busy {service.sum():.2f} s, p99 {np.percentile(plain_lat, 99):.3f} s")
     head = policies["least_work_first"]
     out = {"requests": n_req, "lambda_per_s": lam, "tmax_range": list(tmax), "dt": 0.002,
@@ -622,8 +673,9 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4
         return out
     # saturated: every request queued within a fraction of a second
     arr_sat = poisson_arrivals(n_req, lam_saturated)
-    lat_sat, fig_sat = served(round_pairs, arr_sat)
-    plain_sat = fifo_latency(arr_sat)
+    lat_sat, fig_sat = serve_realtime(ctx, reqs, arr_sat, budget_gb, max_active, round_pairs,
+                                      grid, cells)
+    plain_sat = fifo_latency(arr_sat, service)
     saturated = {"lambda_per_s": lam_saturated, "policy": "least_work_first",
                  "served_busy_s": fig_sat["busy_s"], "served_Gpairs_per_s": fig_sat["Gpairs_per_s"],
                  "p50_latency_s": fig_sat["p50_latency_s"],
@@ -635,6 +687,104 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4
         f"p50 {fig_sat['p50_latency_s']:.3f} s, p99 {fig_sat['p99_latency_s']:.3f} s vs FIFO "
         f"p50 {saturated['plain_p50_latency_s']:.3f} s, p99 {saturated['plain_p99_latency_s']:.3f} s")
     out["saturated"] = saturated
+    return out
+
+
+def bench_serving_stream(ctx, n_req, lam, first, last, out_path, fifo=0, budget_gb=8.0,
+                         max_active=64, round_pairs=3.4e10, tmax=(0.2, 2.0)):
+    """Requests [first, last) of the n_req-request serving stream through one
+    server in real time, each submitted at its Poisson arrival offset from
+    request `first` (lam arrivals per second over the whole stream), with
+    the least-remaining-work round schedule.  The whole stream is longer
+    than one run may take, so it runs as consecutive segments whose records
+    --stream-merge combines; requests still running at a segment's end
+    finish in it, so the next segment starts with an empty server.  fifo >
+    0: the segment's first `fifo` requests are also run one at a time with
+    plain launches, for the FIFO latencies of that prefix.  Writes
+    per-request records (arrival, latency, size, steps) to out_path."""
+    import numpy as np
+    import khrr
+    specs = serving_specs(n_req, tmax_lo=tmax[0], tmax_hi=tmax[1])[first:last]
+    arrivals = poisson_arrivals(n_req, lam)
+    log(f"stream: generating requests {first}..{last - 1}")
+    reqs = []
+    for spec in specs:
+        pos, vel, mass, eps = request_bodies(spec)
+        reqs.append(dict(pos=pos, vel=vel, mass=mass, eps=eps, steps=spec["steps"],
+                         n=pos.shape[1]))
+    arr = arrivals[first:last] - arrivals[first]
+    grid = khrr.cube_grid(128, -2.0, 2.0)
+    cells = 128 ** 3
+    log(f"stream: serving {len(reqs)} requests over {arr[-1]:.0f} s of arrivals")
+    lat, fig = serve_realtime(ctx, reqs, arr, budget_gb, max_active, round_pairs, grid, cells,
+                              label="stream: ")
+    log(f"stream: busy {fig['busy_s']:.1f} s, {fig['Gpairs_per_s']:.0f} Gpairs/s, "
+        f"p50 {fig['p50_latency_s']:.2f} s, p99 {fig['p99_latency_s']:.2f} s")
+    rec = {"stream_requests": n_req, "lambda_per_s": lam, "tmax_range": list(tmax), "dt": 0.002,
+           "grid": "128^3 over [-2,2]^3", "budget_GB": budget_gb, "max_active": max_active,
+           "policy": "least_work_first", "round_pairs": round_pairs,
+           "first": first, "last": last, "figures": fig,
+           "r": [s["r"] for s in specs], "kind": [s["kind"] for s in specs],
+           "n": [s["n"] for s in specs], "steps": [s["steps"] for s in specs],
+           "arrival_s": [float(a) for a in arrivals[first:last]],
+           "latency_s": [float(x) for x in lat]}
+    if fifo > 0:
+        k = min(fifo, len(reqs))
+        service = plain_service_times(ctx, reqs[:k], grid, cells, label="stream: ")
+        rec["fifo_service_s"] = [float(x) for x in service]
+        rec["fifo_latency_s"] = [float(x) for x in fifo_latency(arr[:k], service)]
+        log(f"stream: plain FIFO over the first {k}: busy {service.sum():.1f} s")
+    with open(out_path, "w") as f:
+        json.dump(rec, f)
+    return {k: v for k, v in rec.items() if not isinstance(v, list) or k == "tmax_range"}
+
+
+def stream_merge(paths):
+    """Combines the records of consecutive serving-stream segments: latency
+    percentiles over every request, aggregate pair-interactions per second
+    over the segments' busy time, and the FIFO comparison on the prefix."""
+    import numpy as np
+    recs = sorted((json.load(open(p)) for p in paths), key=lambda r: r["first"])
+    for a, b in zip(recs, recs[1:]):
+        if a["last"] != b["first"]:
+            raise SystemExit(f"segments not contiguous: {a['last']} != {b['first']}")
+    lat = np.concatenate([r["latency_s"] for r in recs])
+    n = np.concatenate([r["n"] for r in recs]).astype(np.float64)
+    steps = np.concatenate([r["steps"] for r in recs]).astype(np.float64)
+    pairs = float(np.sum(n * n * (steps + 3)))
+    busy = sum(r["figures"]["busy_s"] for r in recs)
+    span = sum(r["figures"]["makespan_s"] for r in recs)
+    gal = np.concatenate([np.array(r["kind"]) == "galaxy" for r in recs])
+    out = {"requests": int(len(lat)), "first": recs[0]["first"], "last": recs[-1]["last"],
+           "segments": [[r["first"], r["last"]] for r in recs],
+           "lambda_per_s": recs[0]["lambda_per_s"], "tmax_range": recs[0]["tmax_range"],
+           "budget_GB": recs[0]["budget_GB"], "max_active": recs[0]["max_active"],
+           "grid": recs[0]["grid"], "policy": recs[0]["policy"],
+           "galaxy_requests": int(gal.sum()), "pair_interactions": pairs,
+           "busy_s": busy, "makespan_s": span, "Gpairs_per_s_busy": pairs / busy / 1e9,
+           "p50_latency_s": float(np.percentile(lat, 50)),
+           "p90_latency_s": float(np.percentile(lat, 90)),
+           "p99_latency_s": float(np.percentile(lat, 99)),
+           "max_latency_s": float(np.max(lat)),
+           "p50_latency_galaxy_s": float(np.percentile(lat[gal], 50)) if gal.any() else None,
+           "p99_latency_galaxy_s": float(np.percentile(lat[gal], 99)) if gal.any() else None}
+    fifo = [r for r in recs if "fifo_latency_s" in r]
+    if fifo:
+        r = fifo[0]
+        k = len(r["fifo_latency_s"])
+        served = np.array(r["latency_s"][:k])
+        fl = np.array(r["fifo_latency_s"])
+        sv = np.array(r["fifo_service_s"])
+        nn = np.array(r["n"][:k], np.float64)
+        st = np.array(r["steps"][:k], np.float64)
+        out["fifo_prefix"] = {
+            "requests": [r["first"], r["first"] + k],
+            "plain_busy_s": float(sv.sum()),
+            "plain_Gpairs_per_s": float(np.sum(nn * nn * (st + 3)) / sv.sum() / 1e9),
+            "plain_fifo_p50_latency_s": float(np.percentile(fl, 50)),
+            "plain_fifo_p99_latency_s": float(np.percentile(fl, 99)),
+            "served_p50_latency_s": float(np.percentile(served, 50)),
+            "served_p99_latency_s": float(np.percentile(served, 99))}
     return out
 
 
@@ -785,6 +935,9 @@ def main():
         print("bench.py: this build measures one GPU only (no multi-GPU path is built); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.stream_merge:
+        print(json.dumps(stream_merge(args.stream_merge.split(","))), flush=True)
+        return
     import numpy as np
     import torch
     import ics
@@ -793,6 +946,21 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.stream_range:
+        first, last = (int(v) for v in args.stream_range.split(","))
+        if not 0 <= first < last <= args.stream_requests:
+            raise SystemExit(f"--stream-range {args.stream_range} outside the stream")
+        ctx = khrr.Context(0)
+        if args.tune_cache and os.path.exists(args.tune_cache):
+            try:
+                ctx.tune_load(args.tune_cache)
+            except khrr.KhrrError as e:
+                log(f"tune cache not used: {e}")
+        tmax = tuple(float(v) for v in args.serve_tmax.split(","))
+        out = bench_serving_stream(ctx, args.stream_requests, args.serve_lambda, first, last,
+                                   args.stream_out, fifo=args.stream_fifo, tmax=tmax)
+        print(json.dumps(out), flush=True)
+        return
     if args.calib_copy:
         # a plain copy of known size: 512 MiB read + 512 MiB written
         a = torch.ones(128 * 1024 * 1024, device="cuda:0")
