@@ -10,18 +10,20 @@
 //  1. assign:  cell index per particle from FP64 positions, same arithmetic
 //              as the reference (subtract, IEEE divide, truncate with GPU
 //              cvt saturation, clamp; z clamped with size[1]; y stride sz).
-//  2. sort:    stable LSD radix sort of (cell, particle index), 8-bit
-//              digits; per pass a tile histogram, one scan block per digit
-//              row, and a scatter whose in-block ranks come from 64-lane
-//              ballot matching, so equal cells keep index order.
+//  2. sort:    stable LSD radix sort of (cell, particle index) by the
+//              cell's 512-cell block (cell >> 9: 15 bits at 256^3, two
+//              8-bit passes); per pass a tile histogram, one scan block per
+//              digit row, and a scatter whose in-block ranks come from
+//              64-lane ballot matching, so a block keeps index order.
 //  3. cells:   one 512-thread block per 512-cell block of the reference.
-//              Two binary searches on the sorted cells give the number of
+//              Binary searches on the sorted keys give the number of
 //              particles before the block (= the reference's block-granular
 //              start_id) and through it (= maxPBC); an LDS histogram and
-//              scan give per-cell counts and exact exclusive offsets;
-//              id_in_cell is the sorted position minus the cell's offset,
-//              and each cell adds its particles' FP64 masses in index order
-//              from chunks of the block's masses gathered into LDS.
+//              scan give per-cell counts and exact exclusive offsets; a
+//              stable in-block rank by cell (ballot groups per wave, byte
+//              counters per wave in LDS) gives id_in_cell, the index-order
+//              rank, and each cell adds its particles' FP64 masses in index
+//              order from LDS.
 // Every cell is written exactly once, so no output needs clearing first.
 #include <algorithm>
 #include <climits>
@@ -38,6 +40,8 @@ constexpr int kSortThreads = 256;  // == kRadix: one digit per thread
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kCellBlock = KHRR_BIN_BLOCK;  // 512 cells, BLOCK_SIZE of common.cuh:8
+constexpr int kBlockShift = 9;               // cell >> 9 is the cell's 512-cell block
+static_assert((1 << kBlockShift) == kCellBlock, "block shift");
 
 struct GridArgs {
   double min0, min1, min2;
@@ -128,7 +132,9 @@ __global__ __launch_bounds__(256) void scan_rows(int ntiles, int* __restrict__ h
 // Stable scatter of one digit pass.  Items are visited in index order
 // (round-major, then wave, then lane); a lane's rank among equal digits of
 // its wave comes from eight ballots, and the LDS counters carry the running
-// per-digit position across waves and rounds.
+// per-digit position across waves and rounds.  Every round's key and value
+// are loaded before the first round, so the rounds' global loads are in
+// flight together instead of one per pair of barriers.
 __global__ __launch_bounds__(kSortThreads) void radix_scatter(
     int n, const int* __restrict__ keys_in, const int* __restrict__ vals_in, int shift,
     int ntiles, const int* __restrict__ offs, const int* __restrict__ totals,
@@ -148,11 +154,19 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter(
   __syncthreads();
   const unsigned long long lt = (1ull << lane) - 1ull;
   const int base = blockIdx.x * kSortTile;
+  int keys[kSortItems], vals[kSortItems];
+#pragma unroll
+  for (int it = 0; it < kSortItems; ++it) {
+    const int idx = base + it * kSortThreads + tid;
+    keys[it] = idx < n ? keys_in[idx] : 0;
+    vals[it] = idx < n ? (vals_in ? vals_in[idx] : idx) : 0;
+  }
+#pragma unroll
   for (int it = 0; it < kSortItems; ++it) {
     const int idx = base + it * kSortThreads + tid;
     const bool valid = idx < n;
-    const int key = valid ? keys_in[idx] : 0;
-    const int val = valid ? (vals_in ? vals_in[idx] : idx) : 0;
+    const int key = keys[it];
+    const int val = vals[it];
     const int d = (key >> shift) & (kRadix - 1);
     unsigned long long peers = __ballot(valid);
 #pragma unroll
@@ -195,45 +209,127 @@ __device__ __forceinline__ int lower_bound_dev(const int* __restrict__ a, int n,
   return lo;
 }
 
+// First index of a (sorted by cell >> 9) with a[i] >= key: eight-way steps
+// (seven independent probes per step, so the chain of dependent loads is
+// about a third of a binary search's), then a binary search of the last
+// few entries.
+__device__ __forceinline__ int lower_bound8(const int* __restrict__ a, int n, long long key) {
+  int lo = 0, hi = n;  // the answer lies in [lo, hi]
+  while (hi - lo > 16) {
+    const int step = (hi - lo) >> 3;  // >= 2: probes lo + step .. lo + 7 step < hi
+    int v[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) v[j] = a[lo + (j + 1) * step];
+    int below = 0;  // probes < key: a prefix of the seven, as a is sorted
+#pragma unroll
+    for (int j = 0; j < 7; ++j) below += (long long)v[j] < key ? 1 : 0;
+    const int nlo = below == 0 ? lo : lo + below * step + 1;
+    const int nhi = below == 7 ? hi : lo + (below + 1) * step;
+    lo = nlo;
+    hi = nhi;
+  }
+  return lo + lower_bound_dev(a + lo, hi - lo, key);
+}
+
 // bounds[b] = number of sorted particles with cell < b*512, for b in [0, B]:
-// all block boundaries found by independent binary searches in one launch.
+// all block boundaries found by independent searches in one launch.
 __global__ __launch_bounds__(256) void block_bounds(int n, int nb, const int* __restrict__ skeys,
                                                     int* __restrict__ bounds) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b <= nb) bounds[b] = lower_bound_dev(skeys, n, (long long)b * kCellBlock);
+  if (b <= nb) bounds[b] = lower_bound8(skeys, n, (long long)b * kCellBlock);
 }
 
-// One workgroup per 512-cell block of the reference.  Empty blocks (most
-// of a fine grid) are four coalesced stores per cell, written non-temporal:
-// they are most of the kernel's traffic (20 B a cell) and would otherwise
-// evict from L2 the sorted keys, indices and masses the occupied blocks
-// read.  In an occupied block every thread first loads the cell, index and
-// mass of up to kReg of the block's bodies into registers, so those loads
-// are in flight through the histogram and the scan; an LDS histogram and
-// scan give counts and exact offsets; id_in_cell of the body at sorted
-// position p is p minus its cell's offset (the sort keeps index order
-// within a cell); the masses go to LDS at their sorted positions and each
-// cell adds its own in index order, the FP64 sum of a sequential walk.  A
-// block of more than kReg * 512 bodies repeats the last step per chunk.
-// Stand-alone timing at 256^3 with the headline galaxy: 64.3 us, against
-// 91.6 us with plain stores and one load per step.
+// One workgroup per 512-cell block of the reference.  The sort grouped the
+// bodies by block only, so a block's bodies [lo, hi) arrive in index order
+// but not by cell.
+//  * Empty blocks (most of a fine grid) are four coalesced stores per cell,
+//    written non-temporal: they are most of the kernel's traffic (20 B a
+//    cell) and would otherwise evict from L2 the sorted keys, indices and
+//    masses the occupied blocks read.
+//  * An occupied block loads the cell, index and mass of its first kChunk
+//    bodies into registers (kReg per thread), so those loads are in flight
+//    through the LDS histogram and scan that give counts and exclusive
+//    offsets.
+//  * Ranks: rounds of 512 bodies in index order.  Within a wave, bodies of
+//    the same cell find each other with nine ballots (one per bit of the
+//    block-local cell); each group's first lane records the group size in
+//    a per-wave byte counter, and a body's rank in its cell is the count of
+//    that cell in the rounds before, plus the groups of the waves before
+//    it, plus its place in its own group.  That rank is id_in_cell (the
+//    index-order rank), and offset + rank its position in cell order.
+//  * Mass: each cell adds its bodies' FP64 masses in index order, the sum
+//    of a sequential walk.  A block of at most kChunk bodies drops the
+//    masses it holds in registers into LDS at their cell-order positions;
+//    a larger one writes the cell-order permutation to scratch and sums
+//    chunk by chunk.
+// Every output cell is written exactly once, so nothing needs clearing.
 constexpr int kReg = 4;
+constexpr int kCellWaves = kCellBlock / khrr::kWave;
+
+struct CellRanks {
+  int* run;                                       // bodies per cell in earlier rounds
+  unsigned char (*wc)[kCellWaves][kCellBlock];    // [2] per-wave group sizes
+  const int* exs;                                 // block-relative exclusive offsets
+};
+
+// One round of ranking: body v (block-local cell k, -1 when the lane has
+// none) of this thread, counters of parity `par`.  Every thread of the
+// workgroup calls it (two barriers).  Returns the body's position in cell
+// order (block-relative); -1 for no body.
+__device__ __forceinline__ int rank_round(const CellRanks& cr, int k, int v, int par,
+                                          int* __restrict__ id_in_cell) {
+  const int tid = threadIdx.x;
+  const int lane = tid & (khrr::kWave - 1);
+  const int wave = tid / khrr::kWave;
+  const bool valid = k >= 0;
+  unsigned long long peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 9; ++b) {  // 512 cells per block: nine bits
+    const bool bit = (k >> b) & 1;
+    const unsigned long long bal = __ballot(bit);
+    peers &= bit ? bal : ~bal;
+  }
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  if (valid && (peers & lt) == 0ull) cr.wc[par][wave][k] = (unsigned char)__popcll(peers);
+  __syncthreads();
+  int sp = -1;
+  if (valid) {
+    int rank = cr.run[k] + __popcll(peers & lt);
+    for (int w = 0; w < wave; ++w) rank += cr.wc[par][w][k];
+    if (id_in_cell) id_in_cell[v] = rank;
+    sp = cr.exs[k] + rank;
+  }
+  __syncthreads();
+  // thread t folds cell t's groups of this round into its running count
+  // and clears them; the round after next writes this parity again
+  int add = 0;
+#pragma unroll
+  for (int w = 0; w < kCellWaves; ++w) {
+    add += cr.wc[par][w][tid];
+    cr.wc[par][w][tid] = 0;
+  }
+  cr.run[tid] += add;
+  return sp;
+}
 
 __global__ __launch_bounds__(kCellBlock) void bin_cells(
     long long G, const int* __restrict__ skeys, const int* __restrict__ svals,
     const int* __restrict__ bounds, const double* __restrict__ mass, int* __restrict__ count,
     int* __restrict__ offset, int* __restrict__ start_ref, int* __restrict__ block_incl,
-    int* __restrict__ id_in_cell, double* __restrict__ cell_mass) {
+    int* __restrict__ id_in_cell, double* __restrict__ cell_mass, int* __restrict__ perm) {
   constexpr int kChunk = kReg * kCellBlock;
   __shared__ int cnt[kCellBlock];
   __shared__ int exs[kCellBlock];
-  __shared__ int wsum[kCellBlock / khrr::kWave];
+  __shared__ int run[kCellBlock];
+  __shared__ unsigned char wc[2][kCellWaves][kCellBlock];
+  __shared__ int wsum[kCellWaves];
   __shared__ double msh[kChunk];
   const int tid = threadIdx.x;
-  const long long c0 = (long long)blockIdx.x * kCellBlock;
+  const int blk = blockIdx.x;
+  const long long c0 = (long long)blk * kCellBlock;
   const long long c = c0 + tid;
-  const int lo = bounds[blockIdx.x], hi = bounds[blockIdx.x + 1];
-  if (tid == 0 && block_incl) block_incl[blockIdx.x] = hi;
+  const int lo = bounds[blk], hi = bounds[blk + 1];
+  if (tid == 0 && block_incl) block_incl[blk] = hi;
   if (lo == hi) {
     if (c < G) {
       __builtin_nontemporal_store(0, &count[c]);
@@ -243,6 +339,8 @@ __global__ __launch_bounds__(kCellBlock) void bin_cells(
     }
     return;
   }
+  const int nb = hi - lo;
+  const bool small = nb <= kChunk;  // the same for the whole workgroup
   // the first chunk's cells (relative to the block), indices and masses
   int key[kReg], val[kReg];
   double mv[kReg];
@@ -250,13 +348,16 @@ __global__ __launch_bounds__(kCellBlock) void bin_cells(
   for (int u = 0; u < kReg; ++u) {
     const int p = lo + u * kCellBlock + tid;
     key[u] = p < hi ? skeys[p] - (int)c0 : -1;
-    val[u] = p < hi ? svals[p] : 0;
+    val[u] = p < hi ? (svals ? svals[p] : p) : 0;
   }
-  if (cell_mass) {
+  if (cell_mass && small) {
 #pragma unroll
     for (int u = 0; u < kReg; ++u) mv[u] = key[u] >= 0 ? mass[val[u]] : 0.0;
   }
   cnt[tid] = 0;
+  run[tid] = 0;
+#pragma unroll
+  for (int w = 0; w < kCellWaves; ++w) wc[0][w][tid] = wc[1][w][tid] = 0;
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < kReg; ++u)
@@ -277,43 +378,65 @@ __global__ __launch_bounds__(kCellBlock) void bin_cells(
   __syncthreads();
   int before = 0;
   for (int w = 0; w < wave; ++w) before += wsum[w];
-  const int excl = lo + before + incl - mine;
+  const int excl = before + incl - mine;  // block-relative
   exs[tid] = excl;
   if (c < G) {
     count[c] = mine;
-    if (offset) offset[c] = excl;
+    if (offset) offset[c] = lo + excl;
     if (start_ref) start_ref[c] = lo;
   }
-  __syncthreads();
+  // (the first rank round's first barrier publishes exs)
+  const CellRanks cr{run, wc, exs};
+  const bool want_perm = cell_mass && !small;
+#pragma unroll
+  for (int u = 0; u < kReg; ++u) {
+    if (u * kCellBlock >= nb) break;  // uniform: every thread leaves together
+    const int sp = rank_round(cr, key[u], val[u], u & 1, id_in_cell);
+    if (sp >= 0) {
+      if (cell_mass && small) msh[sp] = mv[u];
+      if (want_perm) perm[lo + sp] = val[u];
+    }
+  }
+  // rounds past the first chunk read their bodies from the sorted keys;
+  // the next round's loads are issued before this round's barriers
+  if (lo + kChunk < hi) {
+    int p = lo + kChunk + tid;
+    int k = p < hi ? skeys[p] - (int)c0 : -1;
+    int v = p < hi ? (svals ? svals[p] : p) : 0;
+    for (int r = kReg, p0 = lo + kChunk; p0 < hi; ++r, p0 += kCellBlock) {
+      const int pn = p0 + kCellBlock + tid;
+      const int kn = pn < hi ? skeys[pn] - (int)c0 : -1;
+      const int vn = pn < hi ? (svals ? svals[pn] : pn) : 0;
+      const int sp = rank_round(cr, k, v, r & 1, id_in_cell);
+      if (sp >= 0 && want_perm) perm[lo + sp] = v;
+      k = kn;
+      v = vn;
+    }
+  }
+  if (!cell_mass) return;
+  __syncthreads();  // msh / perm complete
   double m = 0.0;
-  for (int p0 = lo; p0 < hi; p0 += kChunk) {
-    if (p0 != lo) {  // chunks after the first (blocks of more than kChunk bodies)
+  if (small) {
+    for (int q = excl; q < excl + mine; ++q) m += msh[q];
+  } else {
+    // cell order in chunks of kChunk positions: gather the chunk's masses
+    // into LDS, then each cell adds its part of the chunk in order
+    for (int q0 = 0; q0 < nb; q0 += kChunk) {
+      double mm[kReg];
 #pragma unroll
       for (int u = 0; u < kReg; ++u) {
-        const int p = p0 + u * kCellBlock + tid;
-        key[u] = p < hi ? skeys[p] - (int)c0 : -1;
-        val[u] = p < hi ? svals[p] : 0;
+        const int q = q0 + u * kCellBlock + tid;
+        mm[u] = q < nb ? mass[perm[lo + q]] : 0.0;
       }
-      if (cell_mass) {
 #pragma unroll
-        for (int u = 0; u < kReg; ++u) mv[u] = key[u] >= 0 ? mass[val[u]] : 0.0;
-      }
+      for (int u = 0; u < kReg; ++u) msh[u * kCellBlock + tid] = mm[u];
+      __syncthreads();
+      const int q1 = min(excl + mine, q0 + kChunk);
+      for (int q = max(excl, q0); q < q1; ++q) m += msh[q - q0];
+      __syncthreads();  // msh is refilled by the next chunk
     }
-#pragma unroll
-    for (int u = 0; u < kReg; ++u) {
-      if (key[u] < 0) continue;
-      const int p = p0 + u * kCellBlock + tid;
-      if (id_in_cell) id_in_cell[val[u]] = p - exs[key[u]];
-      if (cell_mass) msh[u * kCellBlock + tid] = mv[u];
-    }
-    if (!cell_mass) continue;  // the same for the whole workgroup
-    __syncthreads();
-    const int p1 = min(hi, p0 + kChunk);
-    const int q1 = min(excl + mine, p1);
-    for (int q = max(excl, p0); q < q1; ++q) m += msh[q - p0];
-    __syncthreads();  // msh is refilled by the next chunk
   }
-  if (c < G && cell_mass) cell_mass[c] = m;
+  if (c < G) cell_mass[c] = m;
 }
 
 // perm[offset[cell_id[i]] + id_in_cell[i]] = i: the sorted segments back
@@ -558,13 +681,17 @@ int bin_launch(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos, c
 
   GridArgs g{grid->min[0], grid->min[1], grid->min[2], grid->cell[0], grid->cell[1],
              grid->cell[2], grid->size[0], grid->size[1], grid->size[2]};
+  // The sort orders bodies by 512-cell block only (stable, so a block's
+  // bodies stay in index order): digits of cell >> 9.  bin_cells orders
+  // each block by cell.
   int bits = 0;
-  while (bits < 31 && (1ll << bits) < G) ++bits;
+  while (bits < 31 && (1ll << bits) < cblocks) ++bits;
   const int passes = (bits + kRadixBits - 1) / kRadixBits;
 
   Timed tm(timed ? ctx : nullptr, KHRR_K_BIN);
   const int* skeys = cells;
-  const int* svals = nullptr;
+  const int* svals = nullptr;  // identity: one block needs no sort
+  int* perm = ka;              // the ping-pong buffer the sort leaves free
   if (n > 0) {
     hipLaunchKernelGGL(bin_assign, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, n, pos, g,
                        cells);
@@ -573,34 +700,25 @@ int bin_launch(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos, c
     for (int p = 0; p < passes; ++p) {
       int* kout = (p & 1) ? kb : ka;
       int* vout = (p & 1) ? vb : va;
+      const int shift = kBlockShift + p * kRadixBits;
       hipLaunchKernelGGL(radix_hist, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, kin,
-                         p * kRadixBits, ntiles, hist);
+                         shift, ntiles, hist);
       hipLaunchKernelGGL(scan_rows, dim3(kRadix), dim3(256), 0, ctx->stream, ntiles, hist,
                          totals);
       hipLaunchKernelGGL(radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, kin,
-                         vin, p * kRadixBits, ntiles, hist, totals, kout, vout);
+                         vin, shift, ntiles, hist, totals, kout, vout);
       kin = kout;
       vin = vout;
+      perm = (p & 1) ? ka : kb;
     }
     skeys = kin;
     svals = vin;
-    if (passes == 0) {
-      // A single cell: the identity permutation is already sorted.
-      hipLaunchKernelGGL(radix_hist, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, cells,
-                         0, ntiles, hist);
-      hipLaunchKernelGGL(scan_rows, dim3(kRadix), dim3(256), 0, ctx->stream, ntiles, hist,
-                         totals);
-      hipLaunchKernelGGL(radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n,
-                         cells, nullptr, 0, ntiles, hist, totals, ka, va);
-      skeys = ka;
-      svals = va;
-    }
   }
   hipLaunchKernelGGL(block_bounds, dim3(ceil_div(cblocks + 1, 256)), dim3(256), 0, ctx->stream, n,
                      cblocks, skeys, bounds);
   hipLaunchKernelGGL(bin_cells, dim3(cblocks), dim3(kCellBlock), 0, ctx->stream, G, skeys, svals,
                      bounds, mass, count, offset, start_id_ref, block_incl, id_in_cell,
-                     cell_mass);
+                     cell_mass, perm);
   return check_launch(ctx, "p2mesh kernels");
 }
 
