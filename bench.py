@@ -792,6 +792,7 @@ def host_cpu_info():
     """CPU model, physical cores and logical CPUs of this host from
     /proc/cpuinfo (physical cores: distinct (physical id, core id) pairs)."""
     model, cores, logical = "", set(), 0
+    ident = {}
     try:
         phys = core = None
         with open("/proc/cpuinfo") as f:
@@ -802,6 +803,13 @@ def host_cpu_info():
                     logical += 1
                 elif key == "model name" and not model:
                     model = val
+                elif key in ("vendor_id", "cpu family", "model", "stepping", "cpu MHz") and \
+                        key not in ident:
+                    ident[key] = val
+                elif key == "flags" and "isa" not in ident:
+                    have = set(val.split())
+                    ident["isa"] = " ".join(x for x in ("avx2", "fma", "avx512f", "avx512dq",
+                                                         "avx512_bf16") if x in have)
                 elif key == "physical id":
                     phys = val
                 elif key == "core id":
@@ -813,7 +821,7 @@ def host_cpu_info():
                 cores.add((phys, core))
     except OSError:
         pass
-    return {"cpu_model": model, "physical_cores": len(cores) or None,
+    return {"cpu_model": model, "cpu_id": ident, "physical_cores": len(cores) or None,
             "logical_cpus": logical or None, "affinity_cpus": len(os.sched_getaffinity(0))}
 
 
@@ -1068,14 +1076,17 @@ def main():
              "dLz_rel": d_end["Lz"] / d_start["Lz"] - 1.0,
              "E0": energy(d0)}
     # HBM bytes per launch from the committed counter passes, when they were
-    # taken on the kernel this run uses
+    # taken on the kernel and launch shape this run uses
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         try:
             tf = json.load(open(TRAFFIC_FILE))
-            if tf.get("kernel", "").startswith("sym_kernel" if tune.symmetric else "grav_kernel"):
+            shape = tf.get("launch_shape", {})
+            same = (tf.get("kernel", "").startswith("sym_kernel" if tune.symmetric else "grav_kernel")
+                    and all(getattr(tune, k) == v for k, v in shape.items()))
+            if same:
                 traffic = tf.get("gravity_bytes_per_launch")
-        except (OSError, ValueError):
+        except (OSError, ValueError, AttributeError):
             traffic = None
     result = {
         "metric": METRIC, "value": value, "unit": "G pair-interactions/s", "n_gpus": 1,

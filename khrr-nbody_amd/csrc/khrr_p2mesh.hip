@@ -62,15 +62,20 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// The reference's cell index of one position (p2mesh-kernel.cuh:25-43).
+__device__ __forceinline__ int cell_of(double x, double y, double z, const GridArgs& g) {
+  const int ix = clampi(sat_trunc((x - g.min0) / g.cell0), 0, g.s0 - 1);
+  const int iy = clampi(sat_trunc((y - g.min1) / g.cell1), 0, g.s1 - 1);
+  const int iz = clampi(sat_trunc((z - g.min2) / g.cell2), 0, g.s1 - 1);
+  return iz * g.s0 * g.s1 + iy * g.s2 + ix;
+}
+
 __global__ __launch_bounds__(256) void bin_assign(int n, const double* __restrict__ pos,
                                                   GridArgs g, int* __restrict__ cell) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const size_t N = (size_t)n;
-  const int ix = clampi(sat_trunc((pos[i] - g.min0) / g.cell0), 0, g.s0 - 1);
-  const int iy = clampi(sat_trunc((pos[N + i] - g.min1) / g.cell1), 0, g.s1 - 1);
-  const int iz = clampi(sat_trunc((pos[2 * N + i] - g.min2) / g.cell2), 0, g.s1 - 1);
-  cell[i] = iz * g.s0 * g.s1 + iy * g.s2 + ix;
+  cell[i] = cell_of(pos[i], pos[N + i], pos[2 * N + i], g);
 }
 
 // Per-tile digit histogram, written digit-major: hist[d * ntiles + tile].
@@ -84,6 +89,40 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist(int n, const int* __r
   for (int it = 0; it < kSortItems; ++it) {
     const int idx = base + it * kSortThreads + threadIdx.x;
     if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kRadix - 1)], 1);
+  }
+  __syncthreads();
+  hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// bin_assign fused with the first pass's tile histogram: each thread loads
+// the positions of its kSortItems bodies of the tile before it divides any
+// of them, so the loads are in flight together; then the cells go out and
+// into the tile's digit histogram (hist[d * ntiles + tile]).
+__global__ __launch_bounds__(kSortThreads) void assign_hist(int n, const double* __restrict__ pos,
+                                                            GridArgs g, int* __restrict__ cell,
+                                                            int shift, int ntiles,
+                                                            int* __restrict__ hist) {
+  __shared__ int h[kRadix];
+  h[threadIdx.x] = 0;
+  const size_t N = (size_t)n;
+  const int base = blockIdx.x * kSortTile;
+  double x[kSortItems], y[kSortItems], z[kSortItems];
+#pragma unroll
+  for (int it = 0; it < kSortItems; ++it) {
+    const int i = base + it * kSortThreads + threadIdx.x;
+    x[it] = i < n ? pos[i] : 0.0;
+    y[it] = i < n ? pos[N + i] : 0.0;
+    z[it] = i < n ? pos[2 * N + i] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kSortItems; ++it) {
+    const int i = base + it * kSortThreads + threadIdx.x;
+    if (i < n) {
+      const int c = cell_of(x[it], y[it], z[it], g);
+      cell[i] = c;
+      atomicAdd(&h[(c >> shift) & (kRadix - 1)], 1);
+    }
   }
   __syncthreads();
   hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
@@ -693,16 +732,21 @@ int bin_launch(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos, c
   const int* svals = nullptr;  // identity: one block needs no sort
   int* perm = ka;              // the ping-pong buffer the sort leaves free
   if (n > 0) {
-    hipLaunchKernelGGL(bin_assign, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, n, pos, g,
-                       cells);
+    if (passes == 0)
+      hipLaunchKernelGGL(bin_assign, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, n, pos, g,
+                         cells);
     const int* kin = cells;
     const int* vin = nullptr;
     for (int p = 0; p < passes; ++p) {
       int* kout = (p & 1) ? kb : ka;
       int* vout = (p & 1) ? vb : va;
       const int shift = kBlockShift + p * kRadixBits;
-      hipLaunchKernelGGL(radix_hist, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, kin,
-                         shift, ntiles, hist);
+      if (p == 0)  // the cells and the first pass's tile histogram in one launch
+        hipLaunchKernelGGL(assign_hist, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, pos,
+                           g, cells, shift, ntiles, hist);
+      else
+        hipLaunchKernelGGL(radix_hist, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, kin,
+                           shift, ntiles, hist);
       hipLaunchKernelGGL(scan_rows, dim3(kRadix), dim3(256), 0, ctx->stream, ntiles, hist,
                          totals);
       hipLaunchKernelGGL(radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, kin,
