@@ -98,6 +98,9 @@ def parse():
     ap.add_argument("--stream-out", default="gpurun_out/stream.json")
     ap.add_argument("--stream-fifo", type=int, default=0,
                     help="also run the segment's first K requests one at a time (plain FIFO)")
+    ap.add_argument("--stream-fifo-only", action="store_true",
+                    help="run the --stream-range requests one at a time only (plain FIFO, no "
+                         "served run); --stream-merge takes the record for the FIFO figures")
     ap.add_argument("--stream-merge", default=None,
                     help="comma list of --stream-out files of consecutive segments: print "
                          "their combined figures (no GPU)")
@@ -690,8 +693,8 @@ def bench_serving(ctx, n_req, lam, budget_gb=8.0, max_active=64, round_pairs=3.4
     return out
 
 
-def bench_serving_stream(ctx, n_req, lam, first, last, out_path, fifo=0, budget_gb=8.0,
-                         max_active=64, round_pairs=3.4e10, tmax=(0.2, 2.0)):
+def bench_serving_stream(ctx, n_req, lam, first, last, out_path, fifo=0, fifo_only=False,
+                         budget_gb=8.0, max_active=64, round_pairs=3.4e10, tmax=(0.2, 2.0)):
     """Requests [first, last) of the n_req-request serving stream through one
     server in real time, each submitted at its Poisson arrival offset from
     request `first` (lam arrivals per second over the whole stream), with
@@ -700,8 +703,9 @@ def bench_serving_stream(ctx, n_req, lam, first, last, out_path, fifo=0, budget_
     --stream-merge combines; requests still running at a segment's end
     finish in it, so the next segment starts with an empty server.  fifo >
     0: the segment's first `fifo` requests are also run one at a time with
-    plain launches, for the FIFO latencies of that prefix.  Writes
-    per-request records (arrival, latency, size, steps) to out_path."""
+    plain launches, for the FIFO latencies of that prefix; fifo_only: only
+    that (no served run).  Writes per-request records (arrival, latency,
+    size, steps) to out_path."""
     import numpy as np
     import khrr
     specs = serving_specs(n_req, tmax_lo=tmax[0], tmax_hi=tmax[1])[first:last]
@@ -715,19 +719,23 @@ def bench_serving_stream(ctx, n_req, lam, first, last, out_path, fifo=0, budget_
     arr = arrivals[first:last] - arrivals[first]
     grid = khrr.cube_grid(128, -2.0, 2.0)
     cells = 128 ** 3
-    log(f"stream: serving {len(reqs)} requests over {arr[-1]:.0f} s of arrivals")
-    lat, fig = serve_realtime(ctx, reqs, arr, budget_gb, max_active, round_pairs, grid, cells,
-                              label="stream: ")
-    log(f"stream: busy {fig['busy_s']:.1f} s, {fig['Gpairs_per_s']:.0f} Gpairs/s, "
-        f"p50 {fig['p50_latency_s']:.2f} s, p99 {fig['p99_latency_s']:.2f} s")
     rec = {"stream_requests": n_req, "lambda_per_s": lam, "tmax_range": list(tmax), "dt": 0.002,
            "grid": "128^3 over [-2,2]^3", "budget_GB": budget_gb, "max_active": max_active,
            "policy": "least_work_first", "round_pairs": round_pairs,
-           "first": first, "last": last, "figures": fig,
+           "first": first, "last": last, "fifo_only": bool(fifo_only),
            "r": [s["r"] for s in specs], "kind": [s["kind"] for s in specs],
            "n": [s["n"] for s in specs], "steps": [s["steps"] for s in specs],
-           "arrival_s": [float(a) for a in arrivals[first:last]],
-           "latency_s": [float(x) for x in lat]}
+           "arrival_s": [float(a) for a in arrivals[first:last]]}
+    if not fifo_only:
+        log(f"stream: serving {len(reqs)} requests over {arr[-1]:.0f} s of arrivals")
+        lat, fig = serve_realtime(ctx, reqs, arr, budget_gb, max_active, round_pairs, grid,
+                                  cells, label="stream: ")
+        log(f"stream: busy {fig['busy_s']:.1f} s, {fig['Gpairs_per_s']:.0f} Gpairs/s, "
+            f"p50 {fig['p50_latency_s']:.2f} s, p99 {fig['p99_latency_s']:.2f} s")
+        rec["figures"] = fig
+        rec["latency_s"] = [float(x) for x in lat]
+    else:
+        fifo = len(reqs)
     if fifo > 0:
         k = min(fifo, len(reqs))
         service = plain_service_times(ctx, reqs[:k], grid, cells, label="stream: ")
@@ -744,7 +752,8 @@ def stream_merge(paths):
     percentiles over every request, aggregate pair-interactions per second
     over the segments' busy time, and the FIFO comparison on the prefix."""
     import numpy as np
-    recs = sorted((json.load(open(p)) for p in paths), key=lambda r: r["first"])
+    allrecs = [json.load(open(p)) for p in paths]
+    recs = sorted((r for r in allrecs if not r.get("fifo_only")), key=lambda r: r["first"])
     for a, b in zip(recs, recs[1:]):
         if a["last"] != b["first"]:
             raise SystemExit(f"segments not contiguous: {a['last']} != {b['first']}")
@@ -768,11 +777,15 @@ def stream_merge(paths):
            "max_latency_s": float(np.max(lat)),
            "p50_latency_galaxy_s": float(np.percentile(lat[gal], 50)) if gal.any() else None,
            "p99_latency_galaxy_s": float(np.percentile(lat[gal], 99)) if gal.any() else None}
-    fifo = [r for r in recs if "fifo_latency_s" in r]
+    fifo = [r for r in allrecs if "fifo_latency_s" in r]
     if fifo:
         r = fifo[0]
         k = len(r["fifo_latency_s"])
-        served = np.array(r["latency_s"][:k])
+        # served latencies of the same requests, from the segment records
+        by_r = {rr: lt for seg in recs for rr, lt in zip(seg["r"], seg["latency_s"])}
+        if not all(rr in by_r for rr in r["r"][:k]):
+            raise SystemExit("the FIFO requests are not all in the served segments")
+        served = np.array([by_r[rr] for rr in r["r"][:k]])
         fl = np.array(r["fifo_latency_s"])
         sv = np.array(r["fifo_service_s"])
         nn = np.array(r["n"][:k], np.float64)
@@ -966,7 +979,8 @@ def main():
                 log(f"tune cache not used: {e}")
         tmax = tuple(float(v) for v in args.serve_tmax.split(","))
         out = bench_serving_stream(ctx, args.stream_requests, args.serve_lambda, first, last,
-                                   args.stream_out, fifo=args.stream_fifo, tmax=tmax)
+                                   args.stream_out, fifo=args.stream_fifo,
+                                   fifo_only=args.stream_fifo_only, tmax=tmax)
         print(json.dumps(out), flush=True)
         return
     if args.calib_copy:

@@ -36,13 +36,16 @@ def test_stream_merge_combines_segments(tmp_path):
                "r": list(range(first, last)), "kind": ["plummer"] * (k - 1) + ["galaxy"],
                "n": [1000] * k, "steps": [97] * k, "arrival_s": list(range(first, last)),
                "latency_s": lat}
-        if first == 0:
-            rec["fifo_service_s"] = [2.0, 2.0]
-            rec["fifo_latency_s"] = [2.0, 3.0]
         path = tmp_path / f"s{first}.json"
         path.write_text(json.dumps(rec))
         segs.append(str(path))
-    r = subprocess.run([sys.executable, BENCH, "--stream-merge", ",".join(segs[::-1])],
+    # the plain FIFO run of requests 0 and 1 alone (no served figures)
+    fifo = {"first": 0, "last": 2, "fifo_only": True, "r": [0, 1], "n": [1000, 1000],
+            "steps": [97, 97], "kind": ["plummer", "plummer"], "arrival_s": [0, 1],
+            "fifo_service_s": [2.0, 2.0], "fifo_latency_s": [2.0, 3.0]}
+    (tmp_path / "fifo.json").write_text(json.dumps(fifo))
+    r = subprocess.run([sys.executable, BENCH, "--stream-merge",
+                        ",".join(segs[::-1] + [str(tmp_path / "fifo.json")])],
                        capture_output=True, text=True, check=True)
     out = json.loads(r.stdout.strip().splitlines()[-1])
     assert out["requests"] == 5 and out["segments"] == [[0, 3], [3, 5]]
