@@ -106,6 +106,37 @@ def test_dense_cells_across_mass_chunks(ctx):
     assert np.sort(got["count"])[-3:].min() > 1500
 
 
+def test_crowded_blocks_among_many(ctx):
+    # 160^3 cells make 8000 blocks of 512, far more than stay resident at once;
+    # every 256th block holds three crowded cells (2700 bodies, past the 2048
+    # that fit in registers, so their masses go through the permutation
+    # scratch) among sparsely occupied and empty blocks
+    nx = 160
+    g = _grid(nx, -2.0, 2.0)
+    h = 4.0 / nx
+
+    def centre(ijk):
+        return -2.0 + (np.asarray(ijk, dtype=np.float64) + 0.5) * h
+
+    probe = np.array([centre(v) for v in ((1, 0, 0), (0, 1, 0), (0, 0, 1), (0, 0, 0))]).T
+    cid = _bin_ref(g, probe, np.ones(4))["cell_id"].astype(np.int64)
+    strides = cid[:3] - cid[3]
+    assert sorted(strides.tolist()) == [1, nx, nx * nx]
+    rng = np.random.default_rng(23)
+    parts = [rng.uniform(-2.0, 2.0, (3, 20000))]
+    for blk in range(0, nx ** 3 // 512, 256):
+        for c in blk * 512 + rng.choice(512, 3, replace=False):
+            ijk = [(c // s) % nx for s in strides]
+            parts.append(centre(ijk)[:, None] + rng.uniform(-0.3 * h, 0.3 * h, (3, 900)))
+    pos = np.concatenate(parts, axis=1)
+    pos = pos[:, rng.permutation(pos.shape[1])]
+    mass = 10.0 ** rng.uniform(-6, 2, pos.shape[1])
+    got = _bin_gpu(ctx, g, pos, mass)
+    ref = _bin_ref(g, pos, mass)
+    _check_equal(got, ref)
+    assert (np.diff(got["block_incl"], prepend=0) > 2048).sum() >= 32
+
+
 def test_reference_start_file_fixture(ctx):
     import os
     from conftest import GOLDEN
