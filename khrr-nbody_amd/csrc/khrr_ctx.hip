@@ -149,8 +149,13 @@ int khrr_ctx_create(int device, khrr_ctx** out) {
     delete ctx;
     return khrr::fail(nullptr, KHRR_ENOMEM, "pinned staging allocation failed");
   }
-  if (khrr::noalloc_init(ctx) != 0) {
+  // The second stream exists from the start, so that no call creates it
+  // while its caller captures the context's stream into a graph.
+  if (khrr::noalloc_init(ctx) != 0 || khrr::aux_stream(ctx) != 0) {
     const std::string msg = ctx->err;
+    if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
+    if (ctx->fork_event) (void)hipEventDestroy(ctx->fork_event);
+    if (ctx->join_event) (void)hipEventDestroy(ctx->join_event);
     (void)hipHostFree(ctx->host_stage);
     (void)hipFree(ctx->dev_words);
     delete ctx;

@@ -137,6 +137,25 @@ def test_crowded_blocks_among_many(ctx):
     assert (np.diff(got["block_incl"], prepend=0) > 2048).sum() >= 32
 
 
+def test_fine_grid_outputs_at_odd_offset(ctx):
+    # outputs at an odd element offset of their allocation on a fine grid
+    import torch
+    pos, _, mass, _ = galaxy_system(4000, 8000, seed=5)
+    g = _grid(200, -2.0, 2.0)
+    n, G = pos.shape[1], 200 ** 3
+    cnt = torch.full((G + 1,), -7, device="cuda:0", dtype=torch.int32)
+    cm = torch.full((G + 1,), -7.0, device="cuda:0", dtype=torch.float64)
+    cell_id = torch.full((n,), -7, device="cuda:0", dtype=torch.int32)
+    ctx.p2mesh_bin(g, to_dev(pos, torch.float64), to_dev(mass, torch.float64), cell_id=cell_id,
+                   count=cnt[1:], cell_mass=cm[1:])
+    torch.cuda.synchronize()
+    ref = _bin_ref(g, pos, mass)
+    np.testing.assert_array_equal(cnt[1:].cpu().numpy(), ref["count"])
+    np.testing.assert_array_equal(cm[1:].cpu().numpy(), ref["cell_mass"])
+    np.testing.assert_array_equal(cell_id.cpu().numpy(), ref["cell_id"])
+    assert cnt[0].item() == -7 and cm[0].item() == -7.0
+
+
 def test_reference_start_file_fixture(ctx):
     import os
     from conftest import GOLDEN
