@@ -14,7 +14,9 @@
 //              cell's 512-cell block (cell >> 9: 15 bits at 256^3, two
 //              8-bit passes); per pass a tile histogram, one scan block per
 //              digit row, and a scatter whose in-block ranks come from
-//              64-lane ballot matching, so a block keeps index order.
+//              64-lane ballot matching, so a block keeps index order.  The
+//              sorted (cell, index) pairs are stored interleaved, one 8-byte
+//              store per body.
 //  3. cells:   one 512-thread block per 512-cell block of the reference.
 //              Binary searches on the sorted keys give the number of
 //              particles before the block (= the reference's block-granular
@@ -80,7 +82,7 @@ __global__ __launch_bounds__(256) void bin_assign(int n, const double* __restric
 
 // Per-tile digit histogram, written digit-major: hist[d * ntiles + tile].
 __global__ __launch_bounds__(kSortThreads) void radix_hist(int n, const int* __restrict__ keys,
-                                                           int shift, int ntiles,
+                                                           int stride, int shift, int ntiles,
                                                            int* __restrict__ hist) {
   __shared__ int h[kRadix];
   h[threadIdx.x] = 0;
@@ -88,7 +90,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist(int n, const int* __r
   const int base = blockIdx.x * kSortTile;
   for (int it = 0; it < kSortItems; ++it) {
     const int idx = base + it * kSortThreads + threadIdx.x;
-    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kRadix - 1)], 1);
+    if (idx < n) atomicAdd(&h[(keys[(size_t)idx * stride] >> shift) & (kRadix - 1)], 1);
   }
   __syncthreads();
   hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
@@ -173,11 +175,13 @@ __global__ __launch_bounds__(256) void scan_rows(int ntiles, int* __restrict__ h
 // its wave comes from eight ballots, and the LDS counters carry the running
 // per-digit position across waves and rounds.  Every round's key and value
 // are loaded before the first round, so the rounds' global loads are in
-// flight together instead of one per pair of barriers.
+// flight together instead of one per pair of barriers.  Input: the cells
+// with the index as value (pairs_in 0) or interleaved (cell, index) pairs;
+// output: interleaved pairs, so each body is one 8-byte store.
 __global__ __launch_bounds__(kSortThreads) void radix_scatter(
-    int n, const int* __restrict__ keys_in, const int* __restrict__ vals_in, int shift,
+    int n, const int* __restrict__ keys_in, int pairs_in, int shift,
     int ntiles, const int* __restrict__ offs, const int* __restrict__ totals,
-    int* __restrict__ keys_out, int* __restrict__ vals_out) {
+    int2* __restrict__ kv_out) {
   constexpr int kWaves = kSortThreads / khrr::kWave;
   __shared__ int run[kRadix];
   __shared__ int wcnt[kWaves][kRadix];
@@ -197,8 +201,14 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter(
 #pragma unroll
   for (int it = 0; it < kSortItems; ++it) {
     const int idx = base + it * kSortThreads + tid;
-    keys[it] = idx < n ? keys_in[idx] : 0;
-    vals[it] = idx < n ? (vals_in ? vals_in[idx] : idx) : 0;
+    if (pairs_in) {
+      const int2 kv = idx < n ? reinterpret_cast<const int2*>(keys_in)[idx] : make_int2(0, 0);
+      keys[it] = kv.x;
+      vals[it] = kv.y;
+    } else {
+      keys[it] = idx < n ? keys_in[idx] : 0;
+      vals[it] = idx;
+    }
   }
 #pragma unroll
   for (int it = 0; it < kSortItems; ++it) {
@@ -230,17 +240,17 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter(
     __syncthreads();
     if (valid) {
       const int p = wpre[wave][d] + rank;
-      keys_out[p] = key;
-      vals_out[p] = val;
+      kv_out[p] = make_int2(key, val);
     }
   }
 }
 
-__device__ __forceinline__ int lower_bound_dev(const int* __restrict__ a, int n, long long key) {
+__device__ __forceinline__ int lower_bound_dev(const int* __restrict__ a, int st, int n,
+                                               long long key) {
   int lo = 0, hi = n;
   while (lo < hi) {
     const int mid = lo + ((hi - lo) >> 1);
-    if ((long long)a[mid] < key)
+    if ((long long)a[(size_t)mid * st] < key)
       lo = mid + 1;
     else
       hi = mid;
@@ -252,13 +262,13 @@ __device__ __forceinline__ int lower_bound_dev(const int* __restrict__ a, int n,
 // (seven independent probes per step, so the chain of dependent loads is
 // about a third of a binary search's), then a binary search of the last
 // few entries.
-__device__ __forceinline__ int lower_bound8(const int* __restrict__ a, int n, long long key) {
+__device__ __forceinline__ int lower_bound8(const int* __restrict__ a, int st, int n, long long key) {
   int lo = 0, hi = n;  // the answer lies in [lo, hi]
   while (hi - lo > 16) {
     const int step = (hi - lo) >> 3;  // >= 2: probes lo + step .. lo + 7 step < hi
     int v[7];
 #pragma unroll
-    for (int j = 0; j < 7; ++j) v[j] = a[lo + (j + 1) * step];
+    for (int j = 0; j < 7; ++j) v[j] = a[(size_t)(lo + (j + 1) * step) * st];
     int below = 0;  // probes < key: a prefix of the seven, as a is sorted
 #pragma unroll
     for (int j = 0; j < 7; ++j) below += (long long)v[j] < key ? 1 : 0;
@@ -267,20 +277,22 @@ __device__ __forceinline__ int lower_bound8(const int* __restrict__ a, int n, lo
     lo = nlo;
     hi = nhi;
   }
-  return lo + lower_bound_dev(a + lo, hi - lo, key);
+  return lo + lower_bound_dev(a + (size_t)lo * st, st, hi - lo, key);
 }
 
 // bounds[b] = number of sorted particles with cell < b*512, for b in [0, B]:
-// all block boundaries found by independent searches in one launch.
+// all block boundaries found by independent searches in one launch.  The
+// keys are every st-th int of skeys (st 2 for interleaved pairs).
 __global__ __launch_bounds__(256) void block_bounds(int n, int nb, const int* __restrict__ skeys,
-                                                    int* __restrict__ bounds) {
+                                                    int st, int* __restrict__ bounds) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b <= nb) bounds[b] = lower_bound8(skeys, n, (long long)b * kCellBlock);
+  if (b <= nb) bounds[b] = lower_bound8(skeys, st, n, (long long)b * kCellBlock);
 }
 
 // One workgroup per 512-cell block of the reference.  The sort grouped the
 // bodies by block only, so a block's bodies [lo, hi) arrive in index order
-// but not by cell.
+// but not by cell.  skeys holds interleaved (cell, index) pairs (pairs 1) or,
+// when no sort ran, the cells in index order (pairs 0).
 //  * Empty blocks (most of a fine grid) are four coalesced stores per cell,
 //    written non-temporal: they are most of the kernel's traffic (20 B a
 //    cell) and would otherwise evict from L2 the sorted keys, indices and
@@ -352,7 +364,7 @@ __device__ __forceinline__ int rank_round(const CellRanks& cr, int k, int v, int
 }
 
 __global__ __launch_bounds__(kCellBlock) void bin_cells(
-    long long G, const int* __restrict__ skeys, const int* __restrict__ svals,
+    long long G, const int* __restrict__ skeys, int pairs,
     const int* __restrict__ bounds, const double* __restrict__ mass, int* __restrict__ count,
     int* __restrict__ offset, int* __restrict__ start_ref, int* __restrict__ block_incl,
     int* __restrict__ id_in_cell, double* __restrict__ cell_mass, int* __restrict__ perm) {
@@ -386,8 +398,14 @@ __global__ __launch_bounds__(kCellBlock) void bin_cells(
 #pragma unroll
   for (int u = 0; u < kReg; ++u) {
     const int p = lo + u * kCellBlock + tid;
-    key[u] = p < hi ? skeys[p] - (int)c0 : -1;
-    val[u] = p < hi ? (svals ? svals[p] : p) : 0;
+    if (pairs) {
+      const int2 kv = p < hi ? reinterpret_cast<const int2*>(skeys)[p] : make_int2(0, 0);
+      key[u] = p < hi ? kv.x - (int)c0 : -1;
+      val[u] = kv.y;
+    } else {
+      key[u] = p < hi ? skeys[p] - (int)c0 : -1;
+      val[u] = p < hi ? p : 0;
+    }
   }
   if (cell_mass && small) {
 #pragma unroll
@@ -401,7 +419,8 @@ __global__ __launch_bounds__(kCellBlock) void bin_cells(
 #pragma unroll
   for (int u = 0; u < kReg; ++u)
     if (key[u] >= 0) atomicAdd(&cnt[key[u]], 1);
-  for (int p = lo + kChunk + tid; p < hi; p += kCellBlock) atomicAdd(&cnt[skeys[p] - (int)c0], 1);
+  for (int p = lo + kChunk + tid; p < hi; p += kCellBlock)
+    atomicAdd(&cnt[skeys[(size_t)p << pairs] - (int)c0], 1);
   __syncthreads();
   // Exclusive scan of cnt: wave-inclusive scans, then wave totals.
   const int lane = tid & (khrr::kWave - 1);
@@ -440,12 +459,12 @@ __global__ __launch_bounds__(kCellBlock) void bin_cells(
   // the next round's loads are issued before this round's barriers
   if (lo + kChunk < hi) {
     int p = lo + kChunk + tid;
-    int k = p < hi ? skeys[p] - (int)c0 : -1;
-    int v = p < hi ? (svals ? svals[p] : p) : 0;
+    int k = p < hi ? skeys[(size_t)p << pairs] - (int)c0 : -1;
+    int v = p < hi ? (pairs ? skeys[((size_t)p << 1) + 1] : p) : 0;
     for (int r = kReg, p0 = lo + kChunk; p0 < hi; ++r, p0 += kCellBlock) {
       const int pn = p0 + kCellBlock + tid;
-      const int kn = pn < hi ? skeys[pn] - (int)c0 : -1;
-      const int vn = pn < hi ? (svals ? svals[pn] : pn) : 0;
+      const int kn = pn < hi ? skeys[(size_t)pn << pairs] - (int)c0 : -1;
+      const int vn = pn < hi ? (pairs ? skeys[((size_t)pn << 1) + 1] : pn) : 0;
       const int sp = rank_round(cr, k, v, r & 1, id_in_cell);
       if (sp >= 0 && want_perm) perm[lo + sp] = v;
       k = kn;
@@ -707,10 +726,8 @@ int bin_launch(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos, c
   const int cblocks = (int)ceil_div(G, kCellBlock);
   Carver cv(base);
   int* cells = cv.take<int>(n > 0 ? n : 1);
-  int* ka = cv.take<int>(n > 0 ? n : 1);
-  int* va = cv.take<int>(n > 0 ? n : 1);
-  int* kb = cv.take<int>(n > 0 ? n : 1);
-  int* vb = cv.take<int>(n > 0 ? n : 1);
+  int2* kva = cv.take<int2>(n > 0 ? n : 1);
+  int2* kvb = cv.take<int2>(n > 0 ? n : 1);
   int* hist = cv.take<int>((size_t)kRadix * ntiles);
   int* totals = cv.take<int>((size_t)kRadix);
   int* bounds = cv.take<int>((size_t)cblocks + 1);
@@ -729,38 +746,35 @@ int bin_launch(khrr_ctx* ctx, const khrr_grid* grid, int n, const double* pos, c
 
   Timed tm(timed ? ctx : nullptr, KHRR_K_BIN);
   const int* skeys = cells;
-  const int* svals = nullptr;  // identity: one block needs no sort
-  int* perm = ka;              // the ping-pong buffer the sort leaves free
+  int pairs = 0;                                 // plain keys, identity values
+  int* perm = reinterpret_cast<int*>(kva);       // the pair buffer the sort leaves free
   if (n > 0) {
     if (passes == 0)
       hipLaunchKernelGGL(bin_assign, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, n, pos, g,
                          cells);
     const int* kin = cells;
-    const int* vin = nullptr;
     for (int p = 0; p < passes; ++p) {
-      int* kout = (p & 1) ? kb : ka;
-      int* vout = (p & 1) ? vb : va;
+      int2* kvout = (p & 1) ? kvb : kva;
       const int shift = kBlockShift + p * kRadixBits;
       if (p == 0)  // the cells and the first pass's tile histogram in one launch
         hipLaunchKernelGGL(assign_hist, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, pos,
                            g, cells, shift, ntiles, hist);
       else
         hipLaunchKernelGGL(radix_hist, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, kin,
-                           shift, ntiles, hist);
+                           2, shift, ntiles, hist);
       hipLaunchKernelGGL(scan_rows, dim3(kRadix), dim3(256), 0, ctx->stream, ntiles, hist,
                          totals);
       hipLaunchKernelGGL(radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, ctx->stream, n, kin,
-                         vin, shift, ntiles, hist, totals, kout, vout);
-      kin = kout;
-      vin = vout;
-      perm = (p & 1) ? ka : kb;
+                         p > 0 ? 1 : 0, shift, ntiles, hist, totals, kvout);
+      kin = reinterpret_cast<const int*>(kvout);
+      perm = reinterpret_cast<int*>((p & 1) ? kva : kvb);
     }
     skeys = kin;
-    svals = vin;
+    pairs = passes > 0 ? 1 : 0;
   }
   hipLaunchKernelGGL(block_bounds, dim3(ceil_div(cblocks + 1, 256)), dim3(256), 0, ctx->stream, n,
-                     cblocks, skeys, bounds);
-  hipLaunchKernelGGL(bin_cells, dim3(cblocks), dim3(kCellBlock), 0, ctx->stream, G, skeys, svals,
+                     cblocks, skeys, pairs ? 2 : 1, bounds);
+  hipLaunchKernelGGL(bin_cells, dim3(cblocks), dim3(kCellBlock), 0, ctx->stream, G, skeys, pairs,
                      bounds, mass, count, offset, start_id_ref, block_incl, id_in_cell,
                      cell_mass, perm);
   return check_launch(ctx, "p2mesh kernels");
