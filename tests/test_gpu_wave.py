@@ -97,11 +97,13 @@ def test_zero_stays_zero_and_rotation(ctx):
 
 
 @pytest.mark.parametrize("nx,iters", [(37, 5), (70, 4), (20, 2), (64, 7), (18, 6), (130, 9),
-                                     (33, 12), (96, 10), (5, 8), (1, 4), (2, 5), (3, 6)])
+                                     (33, 12), (96, 10), (5, 8), (1, 4), (2, 5), (3, 6),
+                                     (200, 8), (47, 13), (64, 16), (9, 11)])
 def test_two_step_pass_equals_single_steps(ctx, nx, iters):
-    # khrr_wave_iterate runs pairs of iterations as one two-step pass; the
-    # fields and the buffer roles must equal single steps with the
-    # reference rotation bit for bit (partial tiles in x, y and z included)
+    # khrr_wave_iterate runs pairs of iterations as one two-step pass and a
+    # last odd iteration as a single step; the fields and the buffer roles
+    # must equal single steps with the reference rotation bit for bit
+    # (partial tiles in x, y and z included)
     import khrr
     import torch
     w = khrr.reference_wave(nx, 4.0 / nx)
