@@ -193,7 +193,14 @@ int admit(khrr_server* s, std::vector<Sim*>& admitted) {
       break;
     }
     m->begin = b;
-    KHRR_TRY(upload(s, m));
+    if (const int rc = upload(s, m)) {
+      // stays pending with nothing held, so a later round can admit it
+      (void)hipFree(m->dev);
+      m->dev = nullptr;
+      m->begin = -1;
+      s->ranges.give(b, m->n);
+      return rc;
+    }
     std::vector<double>().swap(m->pos);
     std::vector<double>().swap(m->vel);
     std::vector<double>().swap(m->mass);
