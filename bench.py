@@ -75,6 +75,9 @@ def parse():
                     help="range of the requests' integration times tmax (uniform), lo,hi")
     ap.add_argument("--serve-saturated", type=float, default=50.0,
                     help="arrival rate of the saturated serving run (0: skip it)")
+    ap.add_argument("--serve-round-pairs", type=float, default=3.4e10,
+                    help="pair-interactions per step of a least-remaining-work round "
+                         "(khrr_server_set_round_pairs)")
     ap.add_argument("--legs", default="binning,stencil,mesh,batched,serving,precision",
                     help="extras to run after the headline (comma list; also batched-run, "
                          "the 512 systems advanced --batched-run-steps steps)")
@@ -980,7 +983,8 @@ def main():
         tmax = tuple(float(v) for v in args.serve_tmax.split(","))
         out = bench_serving_stream(ctx, args.stream_requests, args.serve_lambda, first, last,
                                    args.stream_out, fifo=args.stream_fifo,
-                                   fifo_only=args.stream_fifo_only, tmax=tmax)
+                                   fifo_only=args.stream_fifo_only, tmax=tmax,
+                                   round_pairs=args.serve_round_pairs)
         print(json.dumps(out), flush=True)
         return
     if args.calib_copy:
@@ -1180,7 +1184,8 @@ def main():
         log("serving")
         tmax = tuple(float(v) for v in args.serve_tmax.split(","))
         result["serving"] = bench_serving(ctx, args.serve_requests, args.serve_lambda,
-                                          lam_saturated=args.serve_saturated, tmax=tmax)
+                                          lam_saturated=args.serve_saturated, tmax=tmax,
+                                          round_pairs=args.serve_round_pairs)
         torch.cuda.empty_cache()
     if "precision" in legs and args.precision_steps > 0:
         log("precision form")
