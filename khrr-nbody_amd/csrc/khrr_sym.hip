@@ -109,7 +109,7 @@ __device__ __forceinline__ void sym_block(
     int n, int A, int B, const float* __restrict__ xs, const float* __restrict__ ys,
     const float* __restrict__ zs, const float* __restrict__ m, const float* __restrict__ h,
     const int* __restrict__ sb_flags, float* __restrict__ part,
-    float (&red)[W][4][khrr::kWave]) {
+    float (&red)[2][W][4][khrr::kWave]) {
   static_assert(W >= 4, "the j-side sum uses 4 x 64 threads");
   constexpr int SB = khrr::kWave * IPL * W;  // bodies per superblock
   const int lane = threadIdx.x & (khrr::kWave - 1);
@@ -177,10 +177,14 @@ __device__ __forceinline__ void sym_block(
         if (DO_PHI) fold(acc.ap[k], acc.lp[k], tp[k] * si, COMP);
       }
       // j-side: add the W waves' accumulators in wave order
-      red[wave][0][lane] = bx;
-      red[wave][1][lane] = by;
-      red[wave][2][lane] = bz;
-      red[wave][3][lane] = bp;
+      // two buffers: chunk k writes red[k & 1]; the readers of a buffer
+      // finish before the barrier of chunk k + 1, which every writer of
+      // chunk k + 2 passes first, so one barrier per chunk suffices
+      float (&rb)[W][4][khrr::kWave] = red[((c0 - j0) / khrr::kWave) & 1];
+      rb[wave][0][lane] = bx;
+      rb[wave][1][lane] = by;
+      rb[wave][2][lane] = bz;
+      rb[wave][3][lane] = bp;
       __syncthreads();
       if (threadIdx.x < 4 * khrr::kWave) {
         const int c = threadIdx.x / khrr::kWave;
@@ -188,13 +192,12 @@ __device__ __forceinline__ void sym_block(
         const bool want = c < 3 ? DO_ACC : DO_PHI;
         if (want && c0 + l < j1) {
           float s = 0.f;
-          for (int w = 0; w < W; ++w) s += red[w][c][l];
+          for (int w = 0; w < W; ++w) s += rb[w][c][l];
           const float sj = uni ? mA : 1.f;
           // a_j gets -(sum m_i d_ij / r^3): d_ij points from i to j
           part[((size_t)A * 4 + c) * N + c0 + l] = c < 3 ? -(s * sj) : s * sj;
         }
       }
-      __syncthreads();
     }
   }
 #pragma unroll
@@ -217,7 +220,7 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_kernel(int n, int n_sb,
                                                              const float* __restrict__ h,
                                                              const int* __restrict__ sb_flags,
                                                              float* __restrict__ part) {
-  __shared__ float red[W][4][khrr::kWave];
+  __shared__ float red[2][W][4][khrr::kWave];
   // item -> (A, B), row-major over the upper triangle
   int t = blockIdx.x, A = 0, row = n_sb;
   while (t >= row) {
@@ -238,7 +241,7 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_items_kernel(
     const khrr::SymItem* __restrict__ items, int ld, const float* __restrict__ pos,
     const float* __restrict__ m, const float* __restrict__ h, const int* __restrict__ flags,
     float* __restrict__ part) {
-  __shared__ float red[W][4][khrr::kWave];
+  __shared__ float red[2][W][4][khrr::kWave];
   const khrr::SymItem it = items[blockIdx.x];
   const size_t b = (size_t)it.base;
   sym_block<IPL, W, DO_ACC, DO_PHI, COMP>(it.n, it.A, it.B, pos + b, pos + ld + b,
