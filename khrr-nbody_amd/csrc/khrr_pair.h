@@ -19,6 +19,16 @@ constexpr int kJU = 8;        // j unroll (SGPR budget: 5 arrays x 8 values)
 constexpr int kUniformH = 1;  // every h_j of the range is equal
 constexpr int kUniformM = 2;  // every m_j of the range is equal
 
+// TwoSum with the error term kept in LDS (lo: this lane's slot), for
+// kernels whose registers cannot also hold the error terms.
+__device__ __forceinline__ void fold_lds(float& hi, float* lo, float t) {
+  const float s = hi + t;
+  const float bb = s - hi;
+  const float e = (hi - (s - bb)) + (t - bb);
+  hi = s;
+  *lo += e;
+}
+
 __device__ __forceinline__ void fold(float& hi, float& lo, float t, bool comp) {
   if (comp) {
     // TwoSum (Knuth): hi + t = s + e exactly.
@@ -109,7 +119,9 @@ struct Accum {
 
 // One j-tile [jt, je): pairs into fresh FP32 partials, then one fold of the
 // partials into the running sums.  `flags` (kUniformH | kUniformM, wave
-// uniform) selects the specialised loop.
+// uniform) selects the specialised loop.  lo (compensated form only): the
+// error terms in LDS instead of acc.l*, this lane's slot of a
+// [4 components][IPL][64 lanes] block.
 template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
 __device__ __forceinline__ void tile_step(int jt, int je, int flags, const float* __restrict__ xj,
                                           const float* __restrict__ yj,
@@ -117,7 +129,8 @@ __device__ __forceinline__ void tile_step(int jt, int je, int flags, const float
                                           const float* __restrict__ mj,
                                           const float* __restrict__ hj, const float (&xi)[IPL],
                                           const float (&yi)[IPL], const float (&zi)[IPL],
-                                          const float (&hi)[IPL], Accum<IPL>& acc) {
+                                          const float (&hi)[IPL], Accum<IPL>& acc,
+                                          float* lo = nullptr) {
   float tx[IPL], ty[IPL], tz[IPL], tp[IPL];
 #pragma unroll
   for (int k = 0; k < IPL; ++k) tx[k] = ty[k] = tz[k] = tp[k] = 0.f;
@@ -145,6 +158,15 @@ __device__ __forceinline__ void tile_step(int jt, int je, int flags, const float
   const float scale = um ? mj[jt] : 1.f;
 #pragma unroll
   for (int k = 0; k < IPL; ++k) {
+    if (COMP && lo) {
+      if (DO_ACC) {
+        fold_lds(acc.ax[k], lo + (0 * IPL + k) * 64, tx[k] * scale);
+        fold_lds(acc.ay[k], lo + (1 * IPL + k) * 64, ty[k] * scale);
+        fold_lds(acc.az[k], lo + (2 * IPL + k) * 64, tz[k] * scale);
+      }
+      if (DO_PHI) fold_lds(acc.ap[k], lo + (3 * IPL + k) * 64, tp[k] * scale);
+      continue;
+    }
     if (DO_ACC) {
       fold(acc.ax[k], acc.lx[k], tx[k] * scale, COMP);
       fold(acc.ay[k], acc.ly[k], ty[k] * scale, COMP);

@@ -109,7 +109,7 @@ __device__ __forceinline__ void sym_block(
     int n, int A, int B, const float* __restrict__ xs, const float* __restrict__ ys,
     const float* __restrict__ zs, const float* __restrict__ m, const float* __restrict__ h,
     const int* __restrict__ sb_flags, float* __restrict__ part,
-    float (&red)[2][W][4][khrr::kWave]) {
+    float (&red)[2][W][4][khrr::kWave], float* __restrict__ lo_all) {
   static_assert(W >= 4, "the j-side sum uses 4 x 64 threads");
   constexpr int SB = khrr::kWave * IPL * W;  // bodies per superblock
   const int lane = threadIdx.x & (khrr::kWave - 1);
@@ -133,13 +133,21 @@ __device__ __forceinline__ void sym_block(
   }
   Accum<IPL> acc;
   acc.zero();
+  // compensated form: the TwoSum error terms live in LDS (this lane's slot
+  // of a [4][IPL][64] block per wave), so the registers hold the same
+  // working set as the plain form
+  float* lo = COMP ? lo_all + (size_t)wave * 4 * IPL * khrr::kWave + lane : nullptr;
+  if (COMP) {
+#pragma unroll
+    for (int q = 0; q < 4 * IPL; ++q) lo[q * khrr::kWave] = 0.f;
+  }
 
   if (A == B) {
     // diagonal: one-sided over the superblock's own bodies (j = i included)
     const int j0 = A * SB, j1 = min(n, j0 + SB);
     for (int jt = j0; jt < j1; jt += 256)
       khrr::tile_step<IPL, DO_ACC, DO_PHI, COMP>(jt, min(j1, jt + 256), sb_flags[A], xs, ys, zs,
-                                                 m, h, xi, yi, zi, hi, acc);
+                                                 m, h, xi, yi, zi, hi, acc, lo);
   } else {
     // Only the specialised loop (softening and mass both uniform over A and
     // B) leaves the masses out of the pair terms; the generic loop applies
@@ -169,6 +177,15 @@ __device__ __forceinline__ void sym_block(
       const float si = uni ? mB : 1.f;
 #pragma unroll
       for (int k = 0; k < IPL; ++k) {
+        if (COMP) {
+          if (DO_ACC) {
+            khrr::fold_lds(acc.ax[k], lo + (0 * IPL + k) * khrr::kWave, tx[k] * si);
+            khrr::fold_lds(acc.ay[k], lo + (1 * IPL + k) * khrr::kWave, ty[k] * si);
+            khrr::fold_lds(acc.az[k], lo + (2 * IPL + k) * khrr::kWave, tz[k] * si);
+          }
+          if (DO_PHI) khrr::fold_lds(acc.ap[k], lo + (3 * IPL + k) * khrr::kWave, tp[k] * si);
+          continue;
+        }
         if (DO_ACC) {
           fold(acc.ax[k], acc.lx[k], tx[k] * si, COMP);
           fold(acc.ay[k], acc.ly[k], ty[k] * si, COMP);
@@ -205,11 +222,11 @@ __device__ __forceinline__ void sym_block(
     if (idx[k] >= n) continue;
     const size_t base = (size_t)B * 4 * N + idx[k];
     if (DO_ACC) {
-      part[base] = COMP ? acc.ax[k] + acc.lx[k] : acc.ax[k];
-      part[base + N] = COMP ? acc.ay[k] + acc.ly[k] : acc.ay[k];
-      part[base + 2 * N] = COMP ? acc.az[k] + acc.lz[k] : acc.az[k];
+      part[base] = COMP ? acc.ax[k] + lo[(0 * IPL + k) * khrr::kWave] : acc.ax[k];
+      part[base + N] = COMP ? acc.ay[k] + lo[(1 * IPL + k) * khrr::kWave] : acc.ay[k];
+      part[base + 2 * N] = COMP ? acc.az[k] + lo[(2 * IPL + k) * khrr::kWave] : acc.az[k];
     }
-    if (DO_PHI) part[base + 3 * N] = COMP ? acc.ap[k] + acc.lp[k] : acc.ap[k];
+    if (DO_PHI) part[base + 3 * N] = COMP ? acc.ap[k] + lo[(3 * IPL + k) * khrr::kWave] : acc.ap[k];
   }
 }
 
@@ -221,6 +238,7 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_kernel(int n, int n_sb,
                                                              const int* __restrict__ sb_flags,
                                                              float* __restrict__ part) {
   __shared__ float red[2][W][4][khrr::kWave];
+  __shared__ float lo_all[COMP ? W * 4 * IPL * khrr::kWave : 1];
   // item -> (A, B), row-major over the upper triangle
   int t = blockIdx.x, A = 0, row = n_sb;
   while (t >= row) {
@@ -230,7 +248,7 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_kernel(int n, int n_sb,
   }
   const size_t N = (size_t)n;
   sym_block<IPL, W, DO_ACC, DO_PHI, COMP>(n, A, A + t, pos, pos + N, pos + 2 * N, m, h, sb_flags,
-                                          part, red);
+                                          part, red, lo_all);
 }
 
 // Batched form: workgroup b takes items[b], an (A, B) item of one system of
@@ -242,11 +260,12 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_items_kernel(
     const float* __restrict__ m, const float* __restrict__ h, const int* __restrict__ flags,
     float* __restrict__ part) {
   __shared__ float red[2][W][4][khrr::kWave];
+  __shared__ float lo_all[COMP ? W * 4 * IPL * khrr::kWave : 1];
   const khrr::SymItem it = items[blockIdx.x];
   const size_t b = (size_t)it.base;
   sym_block<IPL, W, DO_ACC, DO_PHI, COMP>(it.n, it.A, it.B, pos + b, pos + ld + b,
                                           pos + 2 * (size_t)ld + b, m + b, h + b,
-                                          flags + it.flag_base, part + it.part, red);
+                                          flags + it.flag_base, part + it.part, red, lo_all);
 }
 
 // acc[i] (+)= sum_X part[X][0..2][i];  phi[i] (-)= sum_X part[X][3][i]
