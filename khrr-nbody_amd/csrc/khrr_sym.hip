@@ -12,18 +12,21 @@
 //    of SB / (64 IPL) waves takes one item (A, B) of the upper triangle,
 //    A <= B;
 //  * off-diagonal items: each wave owns 64*IPL i-bodies of A; the j-bodies of
-//    B stream through in chunks of 64, one per lane, and rotate one lane per
-//    step (DPP wave_ror:1) together with their accumulators, so after 64
-//    steps every lane-i has met every chunk-j once and each j-accumulator is
-//    back in its lane.  The W waves' j-accumulators are then added in LDS in
-//    wave order;
+//    B stream through in chunks of 64.  At step s of a chunk lane l meets
+//    j-body (l + s) mod 64, so after 64 steps every lane-i has met every
+//    chunk-j once.  Plain sums: the wave keeps the chunk in LDS twice over
+//    and adds each step's j-side sums into the body's LDS entry (LDS
+//    loads and stores issue beside the VALU work).  Compensated sums (whose
+//    error terms already take LDS): the j-bodies and their accumulators sit
+//    in registers, one per lane, and rotate one lane per step (DPP
+//    wave_ror:1).  The W waves' j-side sums are then added in wave order;
 //  * diagonal items (A, A) run the one-sided scalar-load loop;
 //  * every (superblock X, body i) partial slot part[X][c][i] is written by
 //    exactly one item, and a second kernel adds the slots in X order: the
 //    result is deterministic and needs no atomics.
 // Per unique pair with uniform mass and softening: 14 VALU ops + one
-// v_rsq_f32 (+6 lane rotations per IPL pairs), i.e. about half of the
-// one-sided cost per directed pair.
+// v_rsq_f32 (plus, per IPL pairs, 3 adds into LDS sums or 6 lane
+// rotations), i.e. about half of the one-sided cost per directed pair.
 #include "khrr_internal.h"
 #include "khrr_pair.h"
 
@@ -99,6 +102,56 @@ __device__ __forceinline__ void chunk64(float& xj, float& yj, float& zj, float& 
   }
 }
 
+// Per-wave LDS of the plain off-diagonal loop: the chunk's j-bodies twice
+// over (entries e and e + 64 both hold body e, so step s reads entry
+// lane + s without wrapping) and two sets of j-side sums, in which entries e
+// and e + 64 together hold body e's.
+struct JWave {
+  float x[2 * khrr::kWave], y[2 * khrr::kWave], z[2 * khrr::kWave];
+  float m[2 * khrr::kWave], h[2 * khrr::kWave];
+  float s[2][4][2 * khrr::kWave];
+};
+
+// 64 steps over one chunk of j-bodies in LDS: at step s lane l pairs its
+// i-bodies with j-body (l + s) mod 64 and adds the step's j-side sums into
+// that body's entry l + s.  Lane l + 1 wrote the entry the step before, so
+// a compiler barrier after each step keeps the loads and stores of the sums
+// in program order, in which the LDS applies one wave's operations.
+template <int IPL, bool DO_ACC, bool DO_PHI, bool UNIH, bool UNIM>
+__device__ __forceinline__ void chunk64_lds(const JWave& J, int par, int lane,
+                                            const float (&xi)[IPL], const float (&yi)[IPL],
+                                            const float (&zi)[IPL], const float (&hi)[IPL],
+                                            const float (&mi)[IPL], float (&tx)[IPL],
+                                            float (&ty)[IPL], float (&tz)[IPL], float (&tp)[IPL]) {
+  const float* px = J.x + lane;
+  const float* py = J.y + lane;
+  const float* pz = J.z + lane;
+  const float* pm = J.m + lane;
+  const float* ph = J.h + lane;
+  float* sx = const_cast<float*>(J.s[par][0]) + lane;
+  float* sy = const_cast<float*>(J.s[par][1]) + lane;
+  float* sz = const_cast<float*>(J.s[par][2]) + lane;
+  float* sp = const_cast<float*>(J.s[par][3]) + lane;
+#pragma unroll 4
+  for (int s = 0; s < khrr::kWave; ++s) {
+    const float xj = px[s], yj = py[s], zj = pz[s];
+    const float mj = UNIM ? 0.f : pm[s];
+    const float hj = UNIH ? 0.f : ph[s];
+    float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
+#pragma unroll
+    for (int k = 0; k < IPL; ++k)
+      pair_sym<DO_ACC, DO_PHI, UNIH, UNIM>(xj, yj, zj, mj, hj, xi[k], yi[k], zi[k], hi[k], mi[k],
+                                           tx[k], ty[k], tz[k], tp[k], bx, by, bz, bp);
+    if (DO_ACC) {
+      sx[s] = sx[s] + bx;
+      sy[s] = sy[s] + by;
+      sz[s] = sz[s] + bz;
+    }
+    if (DO_PHI) sp[s] = sp[s] + bp;
+    asm volatile("" ::: "memory");
+  }
+}
+
 // One (A, B) item of one system of n bodies whose planes start at xs/ys/zs,
 // m, h (any stride between planes); sb_flags[X] are the system's superblock
 // flags and part its slot block, laid out part[(X*4 + c)*n + i] with
@@ -108,8 +161,8 @@ template <int IPL, int W, bool DO_ACC, bool DO_PHI, bool COMP>
 __device__ __forceinline__ void sym_block(
     int n, int A, int B, const float* __restrict__ xs, const float* __restrict__ ys,
     const float* __restrict__ zs, const float* __restrict__ m, const float* __restrict__ h,
-    const int* __restrict__ sb_flags, float* __restrict__ part,
-    float (&red)[2][W][4][khrr::kWave], float* __restrict__ lo_all) {
+    const int* __restrict__ sb_flags, float* __restrict__ part, float* __restrict__ red_mem,
+    JWave* __restrict__ jw, float* __restrict__ lo_all) {
   static_assert(W >= 4, "the j-side sum uses 4 x 64 threads");
   constexpr int SB = khrr::kWave * IPL * W;  // bodies per superblock
   const int lane = threadIdx.x & (khrr::kWave - 1);
@@ -168,12 +221,37 @@ __device__ __forceinline__ void sym_block(
 #pragma unroll
       for (int k = 0; k < IPL; ++k) tx[k] = ty[k] = tz[k] = tp[k] = 0.f;
       float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
-      if (uni)
-        chunk64<IPL, DO_ACC, DO_PHI, true, true>(xj, yj, zj, mj, hj, xi, yi, zi, hh, mi, tx, ty,
-                                                 tz, tp, bx, by, bz, bp);
-      else
-        chunk64<IPL, DO_ACC, DO_PHI, false, false>(xj, yj, zj, mj, hj, xi, yi, zi, hi, mi, tx, ty,
+      const int par = ((c0 - j0) / khrr::kWave) & 1;
+      if (COMP) {
+        if (uni)
+          chunk64<IPL, DO_ACC, DO_PHI, true, true>(xj, yj, zj, mj, hj, xi, yi, zi, hh, mi, tx, ty,
                                                    tz, tp, bx, by, bz, bp);
+        else
+          chunk64<IPL, DO_ACC, DO_PHI, false, false>(xj, yj, zj, mj, hj, xi, yi, zi, hi, mi, tx,
+                                                     ty, tz, tp, bx, by, bz, bp);
+      } else {
+        // this wave's copy of the chunk, and its j-side sums of parity par
+        // cleared (their readers of two chunks ago passed the last barrier)
+        JWave& J = jw[wave];
+        J.x[lane] = J.x[lane + khrr::kWave] = xj;
+        J.y[lane] = J.y[lane + khrr::kWave] = yj;
+        J.z[lane] = J.z[lane + khrr::kWave] = zj;
+        J.m[lane] = J.m[lane + khrr::kWave] = mj;
+        J.h[lane] = J.h[lane + khrr::kWave] = hj;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          if (c < 3 ? !DO_ACC : !DO_PHI) continue;
+          J.s[par][c][lane] = J.s[par][c][lane + khrr::kWave] = 0.f;
+        }
+        // other lanes of this wave read these entries next
+        asm volatile("" ::: "memory");
+        if (uni)
+          chunk64_lds<IPL, DO_ACC, DO_PHI, true, true>(J, par, lane, xi, yi, zi, hh, mi, tx, ty, tz,
+                                                       tp);
+        else
+          chunk64_lds<IPL, DO_ACC, DO_PHI, false, false>(J, par, lane, xi, yi, zi, hi, mi, tx, ty,
+                                                         tz, tp);
+      }
       const float si = uni ? mB : 1.f;
 #pragma unroll
       for (int k = 0; k < IPL; ++k) {
@@ -194,14 +272,17 @@ __device__ __forceinline__ void sym_block(
         if (DO_PHI) fold(acc.ap[k], acc.lp[k], tp[k] * si, COMP);
       }
       // j-side: add the W waves' accumulators in wave order
-      // two buffers: chunk k writes red[k & 1]; the readers of a buffer
+      // two buffers: chunk k writes buffer k & 1; the readers of a buffer
       // finish before the barrier of chunk k + 1, which every writer of
       // chunk k + 2 passes first, so one barrier per chunk suffices
-      float (&rb)[W][4][khrr::kWave] = red[((c0 - j0) / khrr::kWave) & 1];
-      rb[wave][0][lane] = bx;
-      rb[wave][1][lane] = by;
-      rb[wave][2][lane] = bz;
-      rb[wave][3][lane] = bp;
+      float (*rb)[4][khrr::kWave] = nullptr;
+      if (COMP) {
+        rb = reinterpret_cast<float (*)[4][khrr::kWave]>(red_mem) + par * W;
+        rb[wave][0][lane] = bx;
+        rb[wave][1][lane] = by;
+        rb[wave][2][lane] = bz;
+        rb[wave][3][lane] = bp;
+      }
       __syncthreads();
       if (threadIdx.x < 4 * khrr::kWave) {
         const int c = threadIdx.x / khrr::kWave;
@@ -209,7 +290,8 @@ __device__ __forceinline__ void sym_block(
         const bool want = c < 3 ? DO_ACC : DO_PHI;
         if (want && c0 + l < j1) {
           float s = 0.f;
-          for (int w = 0; w < W; ++w) s += rb[w][c][l];
+          for (int w = 0; w < W; ++w)
+            s += COMP ? rb[w][c][l] : jw[w].s[par][c][l] + jw[w].s[par][c][l + khrr::kWave];
           const float sj = uni ? mA : 1.f;
           // a_j gets -(sum m_i d_ij / r^3): d_ij points from i to j
           part[((size_t)A * 4 + c) * N + c0 + l] = c < 3 ? -(s * sj) : s * sj;
@@ -237,7 +319,8 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_kernel(int n, int n_sb,
                                                              const float* __restrict__ h,
                                                              const int* __restrict__ sb_flags,
                                                              float* __restrict__ part) {
-  __shared__ float red[2][W][4][khrr::kWave];
+  __shared__ float red[COMP ? 2 * W * 4 * khrr::kWave : 1];
+  __shared__ JWave jw[COMP ? 1 : W];
   __shared__ float lo_all[COMP ? W * 4 * IPL * khrr::kWave : 1];
   // item -> (A, B), row-major over the upper triangle
   int t = blockIdx.x, A = 0, row = n_sb;
@@ -248,7 +331,7 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_kernel(int n, int n_sb,
   }
   const size_t N = (size_t)n;
   sym_block<IPL, W, DO_ACC, DO_PHI, COMP>(n, A, A + t, pos, pos + N, pos + 2 * N, m, h, sb_flags,
-                                          part, red, lo_all);
+                                          part, red, jw, lo_all);
 }
 
 // Batched form: workgroup b takes items[b], an (A, B) item of one system of
@@ -259,13 +342,14 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_items_kernel(
     const khrr::SymItem* __restrict__ items, int ld, const float* __restrict__ pos,
     const float* __restrict__ m, const float* __restrict__ h, const int* __restrict__ flags,
     float* __restrict__ part) {
-  __shared__ float red[2][W][4][khrr::kWave];
+  __shared__ float red[COMP ? 2 * W * 4 * khrr::kWave : 1];
+  __shared__ JWave jw[COMP ? 1 : W];
   __shared__ float lo_all[COMP ? W * 4 * IPL * khrr::kWave : 1];
   const khrr::SymItem it = items[blockIdx.x];
   const size_t b = (size_t)it.base;
   sym_block<IPL, W, DO_ACC, DO_PHI, COMP>(it.n, it.A, it.B, pos + b, pos + ld + b,
                                           pos + 2 * (size_t)ld + b, m + b, h + b,
-                                          flags + it.flag_base, part + it.part, red, lo_all);
+                                          flags + it.flag_base, part + it.part, red, jw, lo_all);
 }
 
 // acc[i] (+)= sum_X part[X][0..2][i];  phi[i] (-)= sum_X part[X][3][i]
