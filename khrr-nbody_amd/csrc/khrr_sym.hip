@@ -15,7 +15,7 @@
 //    B stream through in chunks of 64.  At step s of a chunk lane l meets
 //    j-body (l + s) mod 64, so after 64 steps every lane-i has met every
 //    chunk-j once.  Plain sums: the wave keeps the chunk in LDS twice over
-//    and adds each step's j-side sums into the body's LDS entry (LDS
+//    and accumulates each step's j-side terms onto the body's LDS entry (LDS
 //    loads and stores issue beside the VALU work).  Compensated sums (whose
 //    error terms already take LDS): the j-bodies and their accumulators sit
 //    in registers, one per lane, and rotate one lane per step (DPP
@@ -25,8 +25,9 @@
 //    exactly one item, and a second kernel adds the slots in X order: the
 //    result is deterministic and needs no atomics.
 // Per unique pair with uniform mass and softening: 14 VALU ops + one
-// v_rsq_f32 (plus, per IPL pairs, 3 adds into LDS sums or 6 lane
-// rotations), i.e. about half of the one-sided cost per directed pair.
+// v_rsq_f32 (plus, per IPL pairs, 3 LDS loads and stores of the j-side sums
+// or 6 lane rotations), i.e. about half of the one-sided cost per directed
+// pair.
 #include "khrr_internal.h"
 #include "khrr_pair.h"
 
@@ -113,10 +114,11 @@ struct JWave {
 };
 
 // 64 steps over one chunk of j-bodies in LDS: at step s lane l pairs its
-// i-bodies with j-body (l + s) mod 64 and adds the step's j-side sums into
-// that body's entry l + s.  Lane l + 1 wrote the entry the step before, so
-// a compiler barrier after each step keeps the loads and stores of the sums
-// in program order, in which the LDS applies one wave's operations.
+// i-bodies with j-body (l + s) mod 64, accumulating onto that body's entry
+// l + s (loaded before the step's pairs, stored after them).  Lane l + 1
+// wrote the entry the step before, so a compiler barrier after each step
+// keeps the loads and stores of the sums in program order, in which the LDS
+// applies one wave's operations.
 template <int IPL, bool DO_ACC, bool DO_PHI, bool UNIH, bool UNIM>
 __device__ __forceinline__ void chunk64_lds(const JWave& J, int par, int lane,
                                             const float (&xi)[IPL], const float (&yi)[IPL],
@@ -137,17 +139,19 @@ __device__ __forceinline__ void chunk64_lds(const JWave& J, int par, int lane,
     const float xj = px[s], yj = py[s], zj = pz[s];
     const float mj = UNIM ? 0.f : pm[s];
     const float hj = UNIH ? 0.f : ph[s];
-    float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
+    // the step's j-side terms go straight onto the body's running sums
+    float bx = DO_ACC ? sx[s] : 0.f, by = DO_ACC ? sy[s] : 0.f, bz = DO_ACC ? sz[s] : 0.f;
+    float bp = DO_PHI ? sp[s] : 0.f;
 #pragma unroll
     for (int k = 0; k < IPL; ++k)
       pair_sym<DO_ACC, DO_PHI, UNIH, UNIM>(xj, yj, zj, mj, hj, xi[k], yi[k], zi[k], hi[k], mi[k],
                                            tx[k], ty[k], tz[k], tp[k], bx, by, bz, bp);
     if (DO_ACC) {
-      sx[s] = sx[s] + bx;
-      sy[s] = sy[s] + by;
-      sz[s] = sz[s] + bz;
+      sx[s] = bx;
+      sy[s] = by;
+      sz[s] = bz;
     }
-    if (DO_PHI) sp[s] = sp[s] + bp;
+    if (DO_PHI) sp[s] = bp;
     asm volatile("" ::: "memory");
   }
 }
