@@ -121,7 +121,8 @@ struct Accum {
 // partials into the running sums.  `flags` (kUniformH | kUniformM, wave
 // uniform) selects the specialised loop.  lo (compensated form only): the
 // error terms in LDS instead of acc.l*, this lane's slot of a
-// [4 components][IPL][64 lanes] block.
+// [components][IPL][64 lanes] block (ax, ay, az, then the potential; the
+// potential alone in row 0).
 template <int IPL, bool DO_ACC, bool DO_PHI, bool COMP>
 __device__ __forceinline__ void tile_step(int jt, int je, int flags, const float* __restrict__ xj,
                                           const float* __restrict__ yj,
@@ -164,7 +165,7 @@ __device__ __forceinline__ void tile_step(int jt, int je, int flags, const float
         fold_lds(acc.ay[k], lo + (1 * IPL + k) * 64, ty[k] * scale);
         fold_lds(acc.az[k], lo + (2 * IPL + k) * 64, tz[k] * scale);
       }
-      if (DO_PHI) fold_lds(acc.ap[k], lo + (3 * IPL + k) * 64, tp[k] * scale);
+      if (DO_PHI) fold_lds(acc.ap[k], lo + ((DO_ACC ? 3 : 0) * IPL + k) * 64, tp[k] * scale);
       continue;
     }
     if (DO_ACC) {

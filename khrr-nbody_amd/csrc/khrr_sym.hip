@@ -12,21 +12,20 @@
 //    of SB / (64 IPL) waves takes one item (A, B) of the upper triangle,
 //    A <= B;
 //  * off-diagonal items: each wave owns 64*IPL i-bodies of A; the j-bodies of
-//    B stream through in chunks of 64.  At step s of a chunk lane l meets
-//    j-body (l + s) mod 64, so after 64 steps every lane-i has met every
-//    chunk-j once.  Plain sums: the wave keeps the chunk in LDS twice over
-//    and accumulates each step's j-side terms onto the body's LDS entry (LDS
-//    loads and stores issue beside the VALU work).  Compensated sums (whose
-//    error terms already take LDS): the j-bodies and their accumulators sit
-//    in registers, one per lane, and rotate one lane per step (DPP
-//    wave_ror:1).  The W waves' j-side sums are then added in wave order;
+//    B stream through in chunks of 64, held in LDS twice over.  At step s of
+//    a chunk lane l meets j-body (l + s) mod 64, so after 64 steps every
+//    lane-i has met every chunk-j once; each wave accumulates the step's
+//    j-side terms onto that body's sums in its own LDS rows (the LDS loads
+//    and stores issue beside the VALU work, where moving the bodies and
+//    sums through the lanes with DPP cost about two FMA slots per move).
+//    The W waves' j-side sums are then added in wave order;
 //  * diagonal items (A, A) run the one-sided scalar-load loop;
 //  * every (superblock X, body i) partial slot part[X][c][i] is written by
 //    exactly one item, and a second kernel adds the slots in X order: the
 //    result is deterministic and needs no atomics.
 // Per unique pair with uniform mass and softening: 14 VALU ops + one
-// v_rsq_f32 (plus, per IPL pairs, 3 LDS loads and stores of the j-side sums
-// or 6 lane rotations), i.e. about half of the one-sided cost per directed
+// v_rsq_f32 (plus, per IPL pairs, 3 LDS loads of the j-body and 3 loads and
+// stores of its sums), i.e. about half of the one-sided cost per directed
 // pair.
 #include "khrr_internal.h"
 #include "khrr_pair.h"
@@ -38,14 +37,6 @@ using khrr::fold;
 using khrr::kUniformH;
 using khrr::kUniformM;
 
-
-__device__ __forceinline__ float ror1(float v) {
-  // DPP wave_ror:1 (dpp_ctrl 0x13C): every lane receives its neighbour's
-  // value (all rows and banks enabled, so `old` is never selected; passing
-  // the source as `old` lets the move happen in place).
-  const int b = __builtin_bit_cast(int, v);
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x13C, 0xF, 0xF, false));
-}
 
 template <bool DO_ACC, bool DO_PHI, bool UNIH, bool UNIM>
 __device__ __forceinline__ void pair_sym(float xj, float yj, float zj, float mj, float hj, float xi,
@@ -75,43 +66,25 @@ __device__ __forceinline__ void pair_sym(float xj, float yj, float zj, float mj,
   }
 }
 
-// 64 rotation steps over one chunk of j-bodies held one per lane.
-template <int IPL, bool DO_ACC, bool DO_PHI, bool UNIH, bool UNIM>
-__device__ __forceinline__ void chunk64(float& xj, float& yj, float& zj, float& mj, float& hj,
-                                        const float (&xi)[IPL], const float (&yi)[IPL],
-                                        const float (&zi)[IPL], const float (&hi)[IPL],
-                                        const float (&mi)[IPL], float (&tx)[IPL],
-                                        float (&ty)[IPL], float (&tz)[IPL], float (&tp)[IPL],
-                                        float& bx, float& by, float& bz, float& bp) {
-#pragma unroll 4
-  for (int s = 0; s < khrr::kWave; ++s) {
-#pragma unroll
-    for (int k = 0; k < IPL; ++k)
-      pair_sym<DO_ACC, DO_PHI, UNIH, UNIM>(xj, yj, zj, mj, hj, xi[k], yi[k], zi[k], hi[k], mi[k],
-                                           tx[k], ty[k], tz[k], tp[k], bx, by, bz, bp);
-    xj = ror1(xj);
-    yj = ror1(yj);
-    zj = ror1(zj);
-    if (!UNIM) mj = ror1(mj);
-    if (!UNIH) hj = ror1(hj);
-    if (DO_ACC) {
-      bx = ror1(bx);
-      by = ror1(by);
-      bz = ror1(bz);
-    }
-    if (DO_PHI) bp = ror1(bp);
-  }
-}
+// Number of j-side sums per body: ax, ay, az and/or the potential, which
+// takes slot 3 next to the acceleration and slot 0 alone.
+template <bool DO_ACC, bool DO_PHI>
+constexpr int kSums = (DO_ACC ? 3 : 0) + (DO_PHI ? 1 : 0);
+template <bool DO_ACC>
+constexpr int kPhiSlot = DO_ACC ? 3 : 0;
 
-// Per-wave LDS of the plain off-diagonal loop: the chunk's j-bodies twice
-// over (entries e and e + 64 both hold body e, so step s reads entry
-// lane + s without wrapping) and two sets of j-side sums, in which entries e
-// and e + 64 together hold body e's.
-struct JWave {
+// One 64-body j chunk in LDS, twice over: entries e and e + 64 both hold
+// body e, so step s reads entry lane + s without wrapping.
+struct JChunk {
   float x[2 * khrr::kWave], y[2 * khrr::kWave], z[2 * khrr::kWave];
   float m[2 * khrr::kWave], h[2 * khrr::kWave];
-  float s[2][4][2 * khrr::kWave];
 };
+
+// LDS floats of the off-diagonal loop's j-side sums: per wave two sets (one
+// per chunk parity) of kSums rows of 128 entries, in which entries e and
+// e + 64 together hold body e's sums.
+template <int W, bool DO_ACC, bool DO_PHI>
+constexpr int kJSumFloats = W * 2 * kSums<DO_ACC, DO_PHI> * 2 * khrr::kWave;
 
 // 64 steps over one chunk of j-bodies in LDS: at step s lane l pairs its
 // i-bodies with j-body (l + s) mod 64, accumulating onto that body's entry
@@ -120,20 +93,21 @@ struct JWave {
 // keeps the loads and stores of the sums in program order, in which the LDS
 // applies one wave's operations.
 template <int IPL, bool DO_ACC, bool DO_PHI, bool UNIH, bool UNIM>
-__device__ __forceinline__ void chunk64_lds(const JWave& J, int par, int lane,
+__device__ __forceinline__ void chunk64_lds(const JChunk& J, float* sums, int lane,
                                             const float (&xi)[IPL], const float (&yi)[IPL],
                                             const float (&zi)[IPL], const float (&hi)[IPL],
                                             const float (&mi)[IPL], float (&tx)[IPL],
                                             float (&ty)[IPL], float (&tz)[IPL], float (&tp)[IPL]) {
+  constexpr int R = 2 * khrr::kWave;
   const float* px = J.x + lane;
   const float* py = J.y + lane;
   const float* pz = J.z + lane;
   const float* pm = J.m + lane;
   const float* ph = J.h + lane;
-  float* sx = const_cast<float*>(J.s[par][0]) + lane;
-  float* sy = const_cast<float*>(J.s[par][1]) + lane;
-  float* sz = const_cast<float*>(J.s[par][2]) + lane;
-  float* sp = const_cast<float*>(J.s[par][3]) + lane;
+  float* sx = sums + 0 * R + lane;
+  float* sy = sums + 1 * R + lane;
+  float* sz = sums + 2 * R + lane;
+  float* sp = sums + kPhiSlot<DO_ACC> * R + lane;
 #pragma unroll 4
   for (int s = 0; s < khrr::kWave; ++s) {
     const float xj = px[s], yj = py[s], zj = pz[s];
@@ -165,8 +139,8 @@ template <int IPL, int W, bool DO_ACC, bool DO_PHI, bool COMP>
 __device__ __forceinline__ void sym_block(
     int n, int A, int B, const float* __restrict__ xs, const float* __restrict__ ys,
     const float* __restrict__ zs, const float* __restrict__ m, const float* __restrict__ h,
-    const int* __restrict__ sb_flags, float* __restrict__ part, float* __restrict__ red_mem,
-    JWave* __restrict__ jw, float* __restrict__ lo_all) {
+    const int* __restrict__ sb_flags, float* __restrict__ part, JChunk (&jc)[2],
+    float* __restrict__ jsum, float* __restrict__ lo_all) {
   static_assert(W >= 4, "the j-side sum uses 4 x 64 threads");
   constexpr int SB = khrr::kWave * IPL * W;  // bodies per superblock
   const int lane = threadIdx.x & (khrr::kWave - 1);
@@ -191,12 +165,14 @@ __device__ __forceinline__ void sym_block(
   Accum<IPL> acc;
   acc.zero();
   // compensated form: the TwoSum error terms live in LDS (this lane's slot
-  // of a [4][IPL][64] block per wave), so the registers hold the same
+  // of a [kSums][IPL][64] block per wave), so the registers hold the same
   // working set as the plain form
-  float* lo = COMP ? lo_all + (size_t)wave * 4 * IPL * khrr::kWave + lane : nullptr;
+  constexpr int NS = kSums<DO_ACC, DO_PHI>;
+  constexpr int PS = kPhiSlot<DO_ACC>;
+  float* lo = COMP ? lo_all + (size_t)wave * NS * IPL * khrr::kWave + lane : nullptr;
   if (COMP) {
 #pragma unroll
-    for (int q = 0; q < 4 * IPL; ++q) lo[q * khrr::kWave] = 0.f;
+    for (int q = 0; q < NS * IPL; ++q) lo[q * khrr::kWave] = 0.f;
   }
 
   if (A == B) {
@@ -216,46 +192,59 @@ __device__ __forceinline__ void sym_block(
     float hh[IPL];  // h_i + h_j of the uniform path
 #pragma unroll
     for (int k = 0; k < IPL; ++k) hh[k] = hsum;
-    for (int c0 = j0; c0 < j1; c0 += khrr::kWave) {
-      const int j = c0 + lane;
+    // The chunk's bodies, or the absent-body sentinel past j1 (an absent j
+    // sits at 1e30: every term it gets or gives is exactly 0).
+    auto load_j = [&](int j, float& x, float& y, float& z, float& mm, float& hh_) {
       const bool vj = j < j1;
-      float xj = vj ? xs[j] : 1e30f, yj = vj ? ys[j] : 1e30f, zj = vj ? zs[j] : 1e30f;
-      float mj = vj ? m[j] : 0.f, hj = vj ? h[j] : 1.f;
+      x = vj ? xs[j] : 1e30f;
+      y = vj ? ys[j] : 1e30f;
+      z = vj ? zs[j] : 1e30f;
+      mm = vj ? m[j] : 0.f;
+      hh_ = vj ? h[j] : 1.f;
+    };
+    auto store_j = [&](JChunk& C, float x, float y, float z, float mm, float hh_) {
+      C.x[lane] = C.x[lane + khrr::kWave] = x;
+      C.y[lane] = C.y[lane + khrr::kWave] = y;
+      C.z[lane] = C.z[lane + khrr::kWave] = z;
+      C.m[lane] = C.m[lane + khrr::kWave] = mm;
+      C.h[lane] = C.h[lane + khrr::kWave] = hh_;
+    };
+    // Wave 0 writes chunk k + 1 into buffer (k + 1) & 1 before its steps of
+    // chunk k; the barrier of chunk k publishes it.  The waves last read
+    // that buffer in chunk k - 1, before the barrier wave 0 passed first.
+    // (Wave 0 waits for those loads; the other workgroups' waves on its SIMD
+    // issue meanwhile.)
+    if (wave == 0) {
+      float x, y, z, mm, hh_;
+      load_j(j0 + lane, x, y, z, mm, hh_);
+      store_j(jc[0], x, y, z, mm, hh_);
+    }
+    __syncthreads();
+    for (int c0 = j0; c0 < j1; c0 += khrr::kWave) {
+      const int par = ((c0 - j0) / khrr::kWave) & 1;
+      const bool next = c0 + khrr::kWave < j1;  // uniform
+      if (wave == 0 && next) {
+        float x, y, z, mm, hh_;
+        load_j(c0 + khrr::kWave + lane, x, y, z, mm, hh_);
+        store_j(jc[par ^ 1], x, y, z, mm, hh_);
+      }
+      // this wave's j-side sums of parity par, cleared: their readers of two
+      // chunks ago finished before the last barrier
+      constexpr int R = 2 * khrr::kWave;  // entries per row of sums
+      float* S = jsum + (size_t)(wave * 2 + par) * NS * R;
+#pragma unroll
+      for (int q = 0; q < NS; ++q) S[q * R + lane] = S[q * R + lane + khrr::kWave] = 0.f;
+      // other lanes of this wave read and add to these entries next
+      asm volatile("" ::: "memory");
       float tx[IPL], ty[IPL], tz[IPL], tp[IPL];
 #pragma unroll
       for (int k = 0; k < IPL; ++k) tx[k] = ty[k] = tz[k] = tp[k] = 0.f;
-      float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
-      const int par = ((c0 - j0) / khrr::kWave) & 1;
-      if (COMP) {
-        if (uni)
-          chunk64<IPL, DO_ACC, DO_PHI, true, true>(xj, yj, zj, mj, hj, xi, yi, zi, hh, mi, tx, ty,
-                                                   tz, tp, bx, by, bz, bp);
-        else
-          chunk64<IPL, DO_ACC, DO_PHI, false, false>(xj, yj, zj, mj, hj, xi, yi, zi, hi, mi, tx,
-                                                     ty, tz, tp, bx, by, bz, bp);
-      } else {
-        // this wave's copy of the chunk, and its j-side sums of parity par
-        // cleared (their readers of two chunks ago passed the last barrier)
-        JWave& J = jw[wave];
-        J.x[lane] = J.x[lane + khrr::kWave] = xj;
-        J.y[lane] = J.y[lane + khrr::kWave] = yj;
-        J.z[lane] = J.z[lane + khrr::kWave] = zj;
-        J.m[lane] = J.m[lane + khrr::kWave] = mj;
-        J.h[lane] = J.h[lane + khrr::kWave] = hj;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          if (c < 3 ? !DO_ACC : !DO_PHI) continue;
-          J.s[par][c][lane] = J.s[par][c][lane + khrr::kWave] = 0.f;
-        }
-        // other lanes of this wave read these entries next
-        asm volatile("" ::: "memory");
-        if (uni)
-          chunk64_lds<IPL, DO_ACC, DO_PHI, true, true>(J, par, lane, xi, yi, zi, hh, mi, tx, ty, tz,
-                                                       tp);
-        else
-          chunk64_lds<IPL, DO_ACC, DO_PHI, false, false>(J, par, lane, xi, yi, zi, hi, mi, tx, ty,
-                                                         tz, tp);
-      }
+      if (uni)
+        chunk64_lds<IPL, DO_ACC, DO_PHI, true, true>(jc[par], S, lane, xi, yi, zi, hh, mi, tx, ty,
+                                                     tz, tp);
+      else
+        chunk64_lds<IPL, DO_ACC, DO_PHI, false, false>(jc[par], S, lane, xi, yi, zi, hi, mi, tx,
+                                                       ty, tz, tp);
       const float si = uni ? mB : 1.f;
 #pragma unroll
       for (int k = 0; k < IPL; ++k) {
@@ -265,7 +254,7 @@ __device__ __forceinline__ void sym_block(
             khrr::fold_lds(acc.ay[k], lo + (1 * IPL + k) * khrr::kWave, ty[k] * si);
             khrr::fold_lds(acc.az[k], lo + (2 * IPL + k) * khrr::kWave, tz[k] * si);
           }
-          if (DO_PHI) khrr::fold_lds(acc.ap[k], lo + (3 * IPL + k) * khrr::kWave, tp[k] * si);
+          if (DO_PHI) khrr::fold_lds(acc.ap[k], lo + (PS * IPL + k) * khrr::kWave, tp[k] * si);
           continue;
         }
         if (DO_ACC) {
@@ -275,27 +264,22 @@ __device__ __forceinline__ void sym_block(
         }
         if (DO_PHI) fold(acc.ap[k], acc.lp[k], tp[k] * si, COMP);
       }
-      // j-side: add the W waves' accumulators in wave order
-      // two buffers: chunk k writes buffer k & 1; the readers of a buffer
-      // finish before the barrier of chunk k + 1, which every writer of
-      // chunk k + 2 passes first, so one barrier per chunk suffices
-      float (*rb)[4][khrr::kWave] = nullptr;
-      if (COMP) {
-        rb = reinterpret_cast<float (*)[4][khrr::kWave]>(red_mem) + par * W;
-        rb[wave][0][lane] = bx;
-        rb[wave][1][lane] = by;
-        rb[wave][2][lane] = bz;
-        rb[wave][3][lane] = bp;
-      }
+      // j-side: add the W waves' sums in wave order.  Chunk k adds into
+      // set k & 1; its readers finish before the barrier of chunk k + 1,
+      // which every writer of chunk k + 2 passes first, so one barrier per
+      // chunk suffices.
       __syncthreads();
       if (threadIdx.x < 4 * khrr::kWave) {
         const int c = threadIdx.x / khrr::kWave;
         const int l = threadIdx.x % khrr::kWave;
         const bool want = c < 3 ? DO_ACC : DO_PHI;
         if (want && c0 + l < j1) {
+          const int q = c < 3 ? c : PS;
           float s = 0.f;
-          for (int w = 0; w < W; ++w)
-            s += COMP ? rb[w][c][l] : jw[w].s[par][c][l] + jw[w].s[par][c][l + khrr::kWave];
+          for (int w = 0; w < W; ++w) {
+            const float* Sw = jsum + ((size_t)(w * 2 + par) * NS + q) * R;
+            s += Sw[l] + Sw[l + khrr::kWave];
+          }
           const float sj = uni ? mA : 1.f;
           // a_j gets -(sum m_i d_ij / r^3): d_ij points from i to j
           part[((size_t)A * 4 + c) * N + c0 + l] = c < 3 ? -(s * sj) : s * sj;
@@ -312,7 +296,8 @@ __device__ __forceinline__ void sym_block(
       part[base + N] = COMP ? acc.ay[k] + lo[(1 * IPL + k) * khrr::kWave] : acc.ay[k];
       part[base + 2 * N] = COMP ? acc.az[k] + lo[(2 * IPL + k) * khrr::kWave] : acc.az[k];
     }
-    if (DO_PHI) part[base + 3 * N] = COMP ? acc.ap[k] + lo[(3 * IPL + k) * khrr::kWave] : acc.ap[k];
+    if (DO_PHI)
+      part[base + 3 * N] = COMP ? acc.ap[k] + lo[(PS * IPL + k) * khrr::kWave] : acc.ap[k];
   }
 }
 
@@ -323,9 +308,9 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_kernel(int n, int n_sb,
                                                              const float* __restrict__ h,
                                                              const int* __restrict__ sb_flags,
                                                              float* __restrict__ part) {
-  __shared__ float red[COMP ? 2 * W * 4 * khrr::kWave : 1];
-  __shared__ JWave jw[COMP ? 1 : W];
-  __shared__ float lo_all[COMP ? W * 4 * IPL * khrr::kWave : 1];
+  __shared__ JChunk jc[2];
+  __shared__ float jsum[kJSumFloats<W, DO_ACC, DO_PHI>];
+  __shared__ float lo_all[COMP ? W * kSums<DO_ACC, DO_PHI> * IPL * khrr::kWave : 1];
   // item -> (A, B), row-major over the upper triangle
   int t = blockIdx.x, A = 0, row = n_sb;
   while (t >= row) {
@@ -335,7 +320,7 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_kernel(int n, int n_sb,
   }
   const size_t N = (size_t)n;
   sym_block<IPL, W, DO_ACC, DO_PHI, COMP>(n, A, A + t, pos, pos + N, pos + 2 * N, m, h, sb_flags,
-                                          part, red, jw, lo_all);
+                                          part, jc, jsum, lo_all);
 }
 
 // Batched form: workgroup b takes items[b], an (A, B) item of one system of
@@ -346,14 +331,14 @@ __global__ __launch_bounds__(khrr::kWave* W) void sym_items_kernel(
     const khrr::SymItem* __restrict__ items, int ld, const float* __restrict__ pos,
     const float* __restrict__ m, const float* __restrict__ h, const int* __restrict__ flags,
     float* __restrict__ part) {
-  __shared__ float red[COMP ? 2 * W * 4 * khrr::kWave : 1];
-  __shared__ JWave jw[COMP ? 1 : W];
-  __shared__ float lo_all[COMP ? W * 4 * IPL * khrr::kWave : 1];
+  __shared__ JChunk jc[2];
+  __shared__ float jsum[kJSumFloats<W, DO_ACC, DO_PHI>];
+  __shared__ float lo_all[COMP ? W * kSums<DO_ACC, DO_PHI> * IPL * khrr::kWave : 1];
   const khrr::SymItem it = items[blockIdx.x];
   const size_t b = (size_t)it.base;
   sym_block<IPL, W, DO_ACC, DO_PHI, COMP>(it.n, it.A, it.B, pos + b, pos + ld + b,
                                           pos + 2 * (size_t)ld + b, m + b, h + b,
-                                          flags + it.flag_base, part + it.part, red, jw, lo_all);
+                                          flags + it.flag_base, part + it.part, jc, jsum, lo_all);
 }
 
 // acc[i] (+)= sum_X part[X][0..2][i];  phi[i] (-)= sum_X part[X][3][i]
