@@ -7,8 +7,9 @@
 // transcendental, subtraction and r^2 work per directed pair.
 //
 // Work decomposition (gfx950):
-//  * bodies are cut into superblocks of SB bodies (4096 for n >= 524288,
-//    else 2048, so smaller systems still make enough items); a workgroup
+//  * bodies are cut into superblocks of SB bodies (by default 4096 for
+//    n >= 524288, 2048 from 32768, else 1024, so smaller systems still make
+//    enough items; a tuned shape may pick any of the three); a workgroup
 //    of SB / (64 IPL) waves takes one item (A, B) of the upper triangle,
 //    A <= B;
 //  * off-diagonal items: each wave owns 64*IPL i-bodies of A; the j-bodies of
