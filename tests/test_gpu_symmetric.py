@@ -159,3 +159,27 @@ def test_uniform_mass_mixed_softening_batched(ctx):
         ref_acc, _ = orc.accel_rows(rows, p32, h32, m32)
         s = err_stats(rel_err(acc[:, a:b][:, rows], ref_acc))
         assert s["median"] <= 1e-6 and s["p99"] <= 1e-5, (k, s)
+
+
+@pytest.mark.parametrize("phi", [False, True], ids=["acc", "acc+phi"])
+def test_compensated_ipl8_superblocks(ctx, phi):
+    # 8 i per lane over 2048-body superblocks: the compensated sums' TwoSum
+    # error terms share the LDS with the j chunk and the j-side sums; both
+    # forms within the bar, the compensated one at least as accurate
+    import khrr
+    pos, _, mass, h = galaxy_system(16000, 24000, seed=12)
+    p32, m32, h32 = pos.astype(np.float32), mass.astype(np.float32), h.astype(np.float32)
+    n = p32.shape[1]
+    t = khrr.Tune(8, 4, 1, 256, 1, 2048)
+    acc_c, phi_c = _run(ctx, p32, m32, h32, t, phi=phi, compensated=True)
+    acc_p, _ = _run(ctx, p32, m32, h32, t, phi=phi)
+    rows = np.sort(np.random.default_rng(3).choice(n, size=4096, replace=False))
+    ref_acc, ref_phi = orc.accel_rows(rows, p32, h32, m32)
+    sc = err_stats(rel_err(acc_c[:, rows], ref_acc))
+    sp = err_stats(rel_err(acc_p[:, rows], ref_acc))
+    print("ipl 8 / 2048: compensated", sc, "plain", sp)
+    assert sc["median"] <= 1e-6 and sc["p99"] <= 1e-5, sc
+    assert sp["median"] <= 1e-6 and sp["p99"] <= 1e-5, sp
+    assert sc["median"] <= sp["median"] and sc["p99"] <= sp["p99"]
+    if phi:
+        assert np.max(np.abs(phi_c[rows] - ref_phi) / np.abs(ref_phi)) <= 2e-6
