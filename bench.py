@@ -98,7 +98,8 @@ def parse():
     ap.add_argument("--stream-range", default=None,
                     help="first,last: serve requests [first, last) of the stream in real time "
                          "and write their records to --stream-out (no other leg runs)")
-    ap.add_argument("--stream-out", default="gpurun_out/stream.json")
+    ap.add_argument("--stream-out", default="stream_segment.json",
+                    help="where --stream-range writes its per-request records")
     ap.add_argument("--stream-fifo", type=int, default=0,
                     help="also run the segment's first K requests one at a time (plain FIFO)")
     ap.add_argument("--stream-fifo-only", action="store_true",
